@@ -1,0 +1,174 @@
+// Python-facing entry points of dbscan_torch on PyTorch-ROCm.
+//
+// Mirrors the reference binding (csrc/extension.cpp:26-81): the same module
+// name (_C), function names, argument names, validation order and error
+// messages.  The compute arm is the gfx950 HIP pipeline behind the C-ABI in
+// include/dbscan_hip.h; this file only validates, stages memory through torch's
+// caching allocator, and passes raw device pointers and the current HIP stream.
+//
+// Host (CPU) tensors are clustered on the current HIP device and the labels
+// are copied back, so the result stays on X's device as the reference promises
+// (csrc/extension.cpp:33-45).  There is no CPU compute path: without a HIP
+// device a host tensor raises.
+
+#include <torch/extension.h>
+
+#include <c10/hip/HIPFunctions.h>
+#include <c10/hip/HIPGuard.h>
+#include <c10/hip/HIPStream.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "dbscan_hip.h"
+
+namespace {
+
+// csrc/extension.cpp:26-31 -- same checks, same order, same messages.
+void check_input(const torch::Tensor& X, double eps, int64_t min_samples) {
+  TORCH_CHECK(X.dim() == 2 && X.size(1) == 2, "X must have shape (N, 2)");
+  TORCH_CHECK(X.scalar_type() == torch::kFloat32, "X must be float32");
+  TORCH_CHECK(eps > 0.0, "eps must be > 0");
+  TORCH_CHECK(min_samples >= 1, "min_samples must be >= 1");
+}
+
+void check_status(int rc) {
+  if (rc == DBSCAN_HIP_ERR_HIP) {
+    TORCH_CHECK(false, "dbscan2d: HIP error ", dbscan2d_hip_last_hip_error(), " (",
+                hipGetErrorString(static_cast<hipError_t>(dbscan2d_hip_last_hip_error())), ")");
+  }
+  TORCH_CHECK(rc == DBSCAN_HIP_OK, "dbscan2d: ", dbscan2d_hip_status_string(rc));
+}
+
+struct DeviceRun {
+  torch::Tensor Xd;         // contiguous input on the HIP device
+  torch::Tensor labels;     // int32 labels on the HIP device
+  torch::Tensor workspace;  // uint8 workspace (kept alive by the debug binding)
+};
+
+// Runs the pipeline; stage_ms may be null.  Returns device tensors.
+DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, float* stage_ms) {
+  c10::Device device = X.device();
+  if (!X.is_cuda()) {
+    TORCH_CHECK(c10::hip::device_count() > 0,
+                "dbscan2d: no HIP device is available. This build clusters on an AMD GPU "
+                "(gfx950); pass a tensor on a ROCm device or run on a machine with one.");
+    device = c10::Device(c10::kCUDA, c10::hip::current_device());
+  }
+  c10::hip::HIPGuard guard(device);
+  DeviceRun r;
+  r.Xd = X.to(device).contiguous();
+  const int64_t n = r.Xd.size(0);
+  TORCH_CHECK(n <= INT32_MAX, "dbscan2d: at most 2^31 - 1 points are supported");
+  size_t ws_bytes = 0;
+  check_status(dbscan2d_hip_workspace_bytes(n, &ws_bytes));
+  auto opts = torch::TensorOptions().device(device);
+  r.workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
+  r.labels = torch::empty({n}, opts.dtype(torch::kInt32));
+  hipStream_t stream = c10::hip::getCurrentHIPStream(device.index()).stream();
+  const int rc =
+      stage_ms ? dbscan2d_hip_profile(r.Xd.data_ptr<float>(), n, eps, min_samples,
+                                      r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(),
+                                      ws_bytes, stream, stage_ms)
+               : dbscan2d_hip(r.Xd.data_ptr<float>(), n, eps, min_samples,
+                              r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(), ws_bytes,
+                              stream);
+  check_status(rc);
+  return r;
+}
+
+torch::Tensor finish(const torch::Tensor& X, const torch::Tensor& labels) {
+  return X.is_cuda() ? labels : labels.to(X.device());
+}
+
+}  // namespace
+
+// csrc/extension.cpp:33-45
+torch::Tensor dbscan2d(torch::Tensor X, double eps, int64_t min_samples) {
+  check_input(X, eps, min_samples);
+  if (X.size(0) == 0) {
+    return torch::empty({0}, X.options().dtype(torch::kInt32));
+  }
+  return finish(X, run_device(X, eps, min_samples, nullptr).labels);
+}
+
+// csrc/extension.cpp:52-63: labels plus GPU milliseconds per reference stage.
+std::pair<torch::Tensor, std::map<std::string, float>> dbscan2d_profile(torch::Tensor X,
+                                                                        double eps,
+                                                                        int64_t min_samples) {
+  check_input(X, eps, min_samples);
+  float ms[DBSCAN_HIP_NUM_STAGES] = {};
+  torch::Tensor labels;
+  if (X.size(0) == 0) {
+    labels = torch::empty({0}, X.options().dtype(torch::kInt32));
+  } else {
+    labels = finish(X, run_device(X, eps, min_samples, ms).labels);
+  }
+  std::map<std::string, float> out;
+  for (int i = 0; i < DBSCAN_HIP_NUM_STAGES; ++i) out[dbscan2d_hip_stage_name(i)] = ms[i];
+  return {labels, out};
+}
+
+// Debug variant (the reference's dbscan2d_cuda_debug, dbscan_cuda.cu:777-800):
+// returns the labels and the intermediates the pipeline leaves in its
+// workspace, as device tensors in the library's internal point and cell order:
+//   0 labels (n,) int32             1 info (8,) int32: kxmin, kxmax, kymin, kymax,
+//                                     num_cells, npasses, error, 0
+//   2 perm (n,) int32 (sorted -> original index)
+//   3 xs (n,) float32   4 ys (n,) float32   (sorted order)
+//   5 point_cell (n,) int32          6 is_core (n,) uint8
+//   7 cell_start (C+1,) int32        8 cell_key (C,) int64 (x' << 32 | y')
+//   9 cell_flags (C,) int32 (bit0 has core, bit1 has non-core)
+//  10 cell_ref (C,) int32 (reference cell order index)
+//  11 uf (C,) int32 (root, by reference index)
+//  12 cell_cluster (C,) int32
+std::vector<torch::Tensor> dbscan2d_hip_debug(torch::Tensor X, double eps, int64_t min_samples) {
+  check_input(X, eps, min_samples);
+  TORCH_CHECK(X.size(0) > 0, "debug variant requires n > 0");
+  DeviceRun r = run_device(X, eps, min_samples, nullptr);
+  const int64_t n = r.Xd.size(0);
+  dbscan_hip_layout L;
+  check_status(dbscan2d_hip_layout(n, &L));
+  auto ws = r.workspace;
+  auto slice = [&](size_t off, int64_t count, int64_t elem, torch::ScalarType t) {
+    return ws.narrow(0, static_cast<int64_t>(off), count * elem).view(t);
+  };
+  auto info = slice(L.info, 8, 4, torch::kInt32).clone();
+  const auto info_h = info.cpu();
+  const int64_t C = info_h[4].item<int64_t>();
+  const int64_t npasses = info_h[5].item<int64_t>();
+  const bool in_a = (npasses & 1) != 0;
+  std::vector<torch::Tensor> out;
+  out.push_back(r.labels);
+  out.push_back(info);
+  out.push_back(slice(in_a ? L.idx_a : L.idx_b, n, 4, torch::kInt32).clone());
+  out.push_back(slice(in_a ? L.xs_a : L.xs_b, n, 4, torch::kFloat32).clone());
+  out.push_back(slice(in_a ? L.ys_a : L.ys_b, n, 4, torch::kFloat32).clone());
+  out.push_back(slice(L.point_cell, n, 4, torch::kInt32).clone());
+  out.push_back(slice(L.is_core, n, 1, torch::kUInt8).clone());
+  out.push_back(slice(L.cell_start, C + 1, 4, torch::kInt32).clone());
+  out.push_back(slice(L.cell_key, C, 8, torch::kInt64).clone());
+  out.push_back(slice(L.cell_flags, C, 4, torch::kInt32).clone());
+  out.push_back(slice(L.cell_ref, C, 4, torch::kInt32).clone());
+  out.push_back(slice(L.uf, C, 4, torch::kInt32).clone());
+  out.push_back(slice(L.cell_cluster, C, 4, torch::kInt32).clone());
+  return out;
+}
+
+// csrc/extension.cpp:65-81
+PYBIND11_MODULE(_C, m) {
+  m.doc() = "Native PyTorch-ROCm DBSCAN (2D, grid-hashed) on gfx950.";
+  m.def("dbscan2d", &dbscan2d, pybind11::arg("X"), pybind11::arg("eps"),
+        pybind11::arg("min_samples"),
+        "Run DBSCAN on (N, 2) float32 tensor. Returns int32 labels (-1 = noise) on X's "
+        "device. The clustering runs on the HIP device.");
+  m.def("dbscan2d_profile", &dbscan2d_profile, pybind11::arg("X"), pybind11::arg("eps"),
+        pybind11::arg("min_samples"),
+        "Profile variant: returns (labels, {stage_name: ms}); GPU time per stage.");
+  m.def("dbscan2d_hip_debug", &dbscan2d_hip_debug, pybind11::arg("X"), pybind11::arg("eps"),
+        pybind11::arg("min_samples"),
+        "Debug variant: returns [labels, info, perm, xs, ys, point_cell, is_core, cell_start, "
+        "cell_key, cell_flags, cell_ref, uf, cell_cluster]. Not part of the public API.");
+}

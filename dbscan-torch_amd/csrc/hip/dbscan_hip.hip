@@ -1,0 +1,1112 @@
+// dbscan_hip.hip -- gfx950 kernels and C-ABI launcher for 2-D grid DBSCAN.
+//
+// Labels are bit-identical to the reference CPU path
+// (csrc/cpu/dbscan_grid.hpp:136-458).  Pipeline, all on one stream, no host
+// synchronisation (the cell count stays on the device):
+//
+//   k_bounds     cell-coordinate bounds of the points (per-block partials) and
+//                clearing of the look-back status words
+//   k_hist       digit histograms of every radix pass, in one read of X
+//   k_sort_pass  stable LSD radix pass (x8, passes beyond the key width exit
+//                at once).  Sorts the points themselves, (x, y, index) in
+//                struct-of-arrays form, by a compressed cell key that is
+//                recomputed from (x, y) in every pass, so no key array exists.
+//                Ranking is wave64 ballot matching into per-wave LDS counters;
+//                tile offsets come from a decoupled look-back per digit.
+//   k_cells      cell runs of the sorted points (look-back scan): point->cell,
+//                cell start, cell key
+//   k_ngh        per cell, the 5 window columns as cell-index ranges (cells of
+//                a column are contiguous and ky-ordered), the cell's index in
+//                the reference's cell order, union-find init
+//   k_core       core marking: full cells wholesale, else window counting
+//   k_union      lock-free union-find over core cells (CAS hooks the larger
+//                reference index onto the smaller)
+//   k_flatten    read-only root walk, root flags in reference order
+//   k_rank       exclusive scan of root flags = dense cluster ids
+//   k_cluster    cluster id per cell
+//   k_label      core labels, border assignment with the reference tiebreak,
+//                and the scatter back to input order, fused
+//
+// Internal cell order is (kx signed, ky signed).  The reference orders cells
+// by the signed int64 key (u32(kx) << 32 | u32(ky)) (dbscan_grid.hpp:176-178),
+// i.e. kx signed then ky as uint32; within one column that puts ky >= 0 before
+// ky < 0.  k_ngh computes each cell's reference index (a rotation within its
+// column) and union-find runs on reference indices, so roots are minimum
+// reference indices and the rank of each root is the reference's cluster id
+// (dbscan_grid.hpp:392-398).
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "dbscan_hip.h"
+
+namespace {
+
+constexpr int32_t kNoise = -1;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 8;           // 64-bit keys at most
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kMaxReduceBlocks = 512;
+constexpr int kCellTileCounter = kMaxPasses;      // tile_counter slot of k_cells
+constexpr int kRankTileCounter = kMaxPasses + 1;  // tile_counter slot of k_rank
+constexpr uint32_t kSpinLimit = 1u << 26;         // bounded waits; see ErrFlag
+constexpr uint64_t kCountMask = (1ull << 48) - 1;
+
+static_assert(kSortThreads == kRadix, "one thread per digit in k_sort_pass");
+
+// Error bits left in Params::error when a bounded wait gives up (never
+// expected; checked by the tests through the debug layout).
+enum ErrFlag : int32_t {
+  kErrSortSpin = 1,
+  kErrScanSpin = 2,
+  kErrUfLoop = 4,
+  kErrRange = 8  // a computed index fell outside its array (write skipped)
+};
+
+// First bytes of the workspace.  The first 8 ints are the `info` block of
+// dbscan_hip_layout.
+struct Params {
+  int32_t kxmin, kxmax, kymin, kymax;
+  int32_t num_cells;
+  int32_t npasses;
+  int32_t error;
+  int32_t pad0;
+  // Everything below is cleared by k_bounds at the start of every call.
+  uint32_t tile_counter[16];
+  uint32_t hist[kMaxPasses][kRadix];
+};
+constexpr size_t kParamsZeroBegin = offsetof(Params, tile_counter);
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+// (int32)floor(v), with out-of-range and NaN mapped to INT32_MIN -- what the
+// reference's x86 build produces for dbscan_grid.hpp:162-163.
+__device__ __forceinline__ int32_t cell_coord(float v) {
+  const float f = floorf(v);
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? static_cast<int32_t>(f) : INT32_MIN;
+}
+
+struct KeyMap {
+  uint32_t kxmin, kymin;  // subtracted (mod 2^32) from kx, ky
+  uint32_t by;            // bit width of the y part
+  int npasses;            // radix passes covering bx + by bits (>= 1)
+};
+
+__device__ __forceinline__ uint32_t bit_width(uint32_t v) {
+  return v ? 32u - static_cast<uint32_t>(__clz(static_cast<int>(v))) : 0u;
+}
+
+__device__ __forceinline__ KeyMap make_keymap(int32_t kxmin, int32_t kxmax, int32_t kymin,
+                                              int32_t kymax) {
+  KeyMap m;
+  m.kxmin = static_cast<uint32_t>(kxmin);
+  m.kymin = static_cast<uint32_t>(kymin);
+  const uint32_t bx = bit_width(static_cast<uint32_t>(kxmax) - m.kxmin);
+  m.by = bit_width(static_cast<uint32_t>(kymax) - m.kymin);
+  const int nbits = static_cast<int>(bx + m.by);
+  m.npasses = nbits ? (nbits + kRadixBits - 1) / kRadixBits : 1;
+  return m;
+}
+
+__device__ __forceinline__ KeyMap load_keymap(const Params* p) {
+  return make_keymap(p->kxmin, p->kxmax, p->kymin, p->kymax);
+}
+
+// Relative cell coordinates (x', y') = (kx - kxmin, ky - kymin) of a point.
+__device__ __forceinline__ void rel_cell(float x, float y, float inv_r, const KeyMap& m,
+                                         uint32_t& xr, uint32_t& yr) {
+  xr = static_cast<uint32_t>(cell_coord(x * inv_r)) - m.kxmin;
+  yr = static_cast<uint32_t>(cell_coord(y * inv_r)) - m.kymin;
+}
+
+// Compressed sort key: x' above y', bx + by bits in all.
+__device__ __forceinline__ uint64_t sort_key(float x, float y, float inv_r, const KeyMap& m) {
+  uint32_t xr, yr;
+  rel_cell(x, y, inv_r, m, xr, yr);
+  return (static_cast<uint64_t>(xr) << m.by) | yr;
+}
+
+// Cell key in internal order: (x' << 32 | y').
+__device__ __forceinline__ uint64_t cell_key64(uint32_t xr, uint32_t yr) {
+  return (static_cast<uint64_t>(xr) << 32) | yr;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_acquire(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_release(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan over a block of NT threads; `s_w` holds NT/64 words.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total, uint32_t* s_w) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) s_w[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; ++i) {
+    const uint32_t s = s_w[i];
+    off += (i < w) ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - v;
+}
+
+// Decoupled look-back for a single running total per tile (tags: 1 = tile
+// aggregate, 2 = inclusive prefix).  Called by one thread; returns the
+// exclusive prefix of `tile`.
+__device__ uint64_t lookback_single(uint64_t* status, uint32_t tile, uint64_t aggregate,
+                                    int32_t* error) {
+  constexpr uint64_t kAgg = 1ull << 48, kPre = 2ull << 48;
+  if (tile == 0) {
+    st_release(status, kPre | aggregate);
+    return 0;
+  }
+  st_release(status + tile, kAgg | aggregate);
+  uint64_t excl = 0;
+  int64_t j = static_cast<int64_t>(tile) - 1;
+  uint32_t spins = 0;
+  while (j >= 0) {
+    const uint64_t s = ld_acquire(status + j);
+    const uint64_t tag = s >> 48;
+    if (tag == 0) {
+      if (++spins > kSpinLimit) {
+        atomicOr(error, static_cast<int32_t>(kErrScanSpin));
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += s & kCountMask;
+    if (tag == 2) break;
+    --j;
+  }
+  st_release(status + tile, kPre | (excl + aggregate));
+  return excl;
+}
+
+// Union-find over reference cell indices.  Parents only ever move to smaller
+// indices (dbscan_grid.hpp:80-117), so every walk ends at the component's
+// minimum index.  Loads are agent-scope atomics so that writes by other
+// workgroups (on other XCDs) are seen within the kernel.
+__device__ __forceinline__ int32_t uf_find_ro(const int32_t* uf, int32_t x) {
+  int32_t p = ld_acquire(uf + x);
+  while (p != x) {
+    x = p;
+    p = ld_acquire(uf + x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ int32_t uf_find_halving(int32_t* uf, int32_t x) {
+  int32_t p = ld_acquire(uf + x);
+  while (p != x) {
+    const int32_t gp = ld_acquire(uf + p);
+    if (gp != p) st_release(uf + x, gp);
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__device__ void uf_unite(int32_t* uf, int32_t a, int32_t b, int32_t* error) {
+  for (uint32_t it = 0; it < kSpinLimit; ++it) {
+    a = uf_find_halving(uf, a);
+    b = uf_find_halving(uf, b);
+    if (a == b) return;
+    const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
+    if (atomicCAS(uf + hi, hi, lo) == hi) return;
+  }
+  atomicOr(error, static_cast<int32_t>(kErrUfLoop));
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ out, int64_t n,
+                                                  int32_t v) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * 256)
+    out[i] = v;
+}
+
+__device__ __forceinline__ void block_minmax4(int32_t v[4], int32_t (*s)[4]) {
+  // v = {min, max, min, max}
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    v[0] = min(v[0], __shfl_xor(v[0], d, 64));
+    v[1] = max(v[1], __shfl_xor(v[1], d, 64));
+    v[2] = min(v[2], __shfl_xor(v[2], d, 64));
+    v[3] = max(v[3], __shfl_xor(v[3], d, 64));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[w][k] = v[k];
+  }
+  __syncthreads();
+  v[0] = s[0][0]; v[1] = s[0][1]; v[2] = s[0][2]; v[3] = s[0][3];
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    v[0] = min(v[0], s[i][0]);
+    v[1] = max(v[1], s[i][1]);
+    v[2] = min(v[2], s[i][2]);
+    v[3] = max(v[3], s[i][3]);
+  }
+  __syncthreads();
+}
+
+// Cell-coordinate bounds, one partial per block; also clears the words the
+// later kernels accumulate into (histograms, tile counters, look-back status).
+__global__ __launch_bounds__(256) void k_bounds(const float2* __restrict__ X, int64_t n,
+                                                float inv_r, int4* __restrict__ partials,
+                                                uint4* __restrict__ zero, int64_t zero16) {
+  const int64_t gtid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  for (int64_t z = gtid; z < zero16; z += stride) zero[z] = make_uint4(0, 0, 0, 0);
+
+  int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  for (int64_t i = gtid; i < n; i += stride) {
+    const float2 p = X[i];
+    const int32_t kx = cell_coord(p.x * inv_r), ky = cell_coord(p.y * inv_r);
+    v[0] = min(v[0], kx); v[1] = max(v[1], kx);
+    v[2] = min(v[2], ky); v[3] = max(v[3], ky);
+  }
+  __shared__ int32_t s[4][4];
+  block_minmax4(v, s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = make_int4(v[0], v[1], v[2], v[3]);
+}
+
+// Reduces the bound partials, publishes them, and accumulates the digit
+// histogram of every radix pass.
+__global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int64_t n,
+                                              float inv_r, const int4* __restrict__ partials,
+                                              int nparts, Params* __restrict__ prm) {
+  __shared__ int32_t s_red[4][4];
+  __shared__ uint32_t s_hist[kMaxPasses][kRadix];
+  int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  for (int i = threadIdx.x; i < nparts; i += 256) {
+    const int4 q = partials[i];
+    v[0] = min(v[0], q.x); v[1] = max(v[1], q.y);
+    v[2] = min(v[2], q.z); v[3] = max(v[3], q.w);
+  }
+  block_minmax4(v, s_red);
+  const KeyMap m = make_keymap(v[0], v[1], v[2], v[3]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    prm->kxmin = v[0]; prm->kxmax = v[1]; prm->kymin = v[2]; prm->kymax = v[3];
+    prm->npasses = m.npasses;
+    prm->num_cells = 0;
+    prm->error = 0;
+  }
+  for (int e = threadIdx.x; e < kMaxPasses * kRadix; e += 256) (&s_hist[0][0])[e] = 0u;
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+    const float2 p = X[i];
+    const uint64_t key = sort_key(p.x, p.y, inv_r, m);
+    for (int ps = 0; ps < m.npasses; ++ps)
+      atomicAdd(&s_hist[ps][static_cast<uint32_t>(key >> (kRadixBits * ps)) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m.npasses * kRadix; e += 256) {
+    const uint32_t c = (&s_hist[0][0])[e];
+    if (c) atomicAdd(&prm->hist[0][0] + e, c);
+  }
+}
+
+// One stable LSD radix pass over the digit (key >> 8*pass) & 255.
+// kFirst: read the interleaved input X and emit original indices.
+template <bool kFirst>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(
+    const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
+    const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
+    uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
+    int pass, int64_t n, float inv_r) {
+  const KeyMap m = load_keymap(prm);
+  if (pass >= m.npasses) return;
+
+  __shared__ float s_x[kSortTile];
+  __shared__ float s_y[kSortTile];
+  __shared__ uint32_t s_i[kSortTile];
+  __shared__ uint8_t s_d[kSortTile];
+  __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
+  __shared__ uint32_t s_goff[kRadix];
+  __shared__ uint32_t s_dstart[kRadix];
+  __shared__ uint32_t s_scan[kSortThreads / 64];
+  __shared__ uint32_t s_tile;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+  for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][t] = 0u;
+  if (t == 0) s_tile = atomicAdd(&prm->tile_counter[pass], 1u);
+  uint32_t htot;
+  const uint32_t goff = block_excl_scan<kSortThreads>(prm->hist[pass][t], htot, s_scan);
+  const uint32_t tile = s_tile;
+  const int64_t base = static_cast<int64_t>(tile) * kSortTile;
+  const int64_t wbase = base + static_cast<int64_t>(wave) * (64 * kSortItems);
+  const int shift = kRadixBits * pass;
+
+  float x[kSortItems], y[kSortItems];
+  uint32_t id[kSortItems], dg[kSortItems], rk[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const int64_t e = wbase + k * 64 + lane;
+    x[k] = 0.f; y[k] = 0.f; id[k] = 0u; dg[k] = 0u;
+    if (e < n) {
+      if (kFirst) {
+        const float2 p = X[e];
+        x[k] = p.x; y[k] = p.y; id[k] = static_cast<uint32_t>(e);
+      } else {
+        x[k] = in_x[e]; y[k] = in_y[e]; id[k] = in_i[e];
+      }
+      dg[k] = static_cast<uint32_t>(sort_key(x[k], y[k], inv_r, m) >> shift) & (kRadix - 1);
+    }
+  }
+
+  // Rank within the wave's segment, item by item (stable): lanes holding the
+  // same digit are found with 8 ballots; the lowest such lane bumps the wave's
+  // counter for that digit.
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const bool valid = (wbase + k * 64 + lane) < n;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const bool bit = (dg[k] >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
+    uint32_t old = 0;
+    if (valid) old = s_wcnt[wave][dg[k]];
+    if (valid && below == 0) s_wcnt[wave][dg[k]] = old + static_cast<uint32_t>(__popcll(peers));
+    rk[k] = old + below;
+  }
+  __syncthreads();
+
+  // Thread t owns digit t: wave offsets, tile total, tile-local start.
+  const uint32_t c0 = s_wcnt[0][t], c1 = s_wcnt[1][t], c2 = s_wcnt[2][t], c3 = s_wcnt[3][t];
+  const uint32_t dtot = c0 + c1 + c2 + c3;
+  s_wcnt[0][t] = 0u;
+  s_wcnt[1][t] = c0;
+  s_wcnt[2][t] = c0 + c1;
+  s_wcnt[3][t] = c0 + c1 + c2;
+  const uint64_t tag_agg = static_cast<uint64_t>(2 * pass + 1) << 48;
+  const uint64_t tag_pre = static_cast<uint64_t>(2 * pass + 2) << 48;
+  uint64_t* my_status = status + static_cast<uint64_t>(tile) * kRadix + t;
+  st_release(my_status, (tile == 0 ? tag_pre : tag_agg) | dtot);
+  uint32_t ttot;
+  const uint32_t dstart = block_excl_scan<kSortThreads>(dtot, ttot, s_scan);
+  s_dstart[t] = dstart;
+
+  // Look-back over earlier tiles for digit t.  Status words of earlier passes
+  // carry smaller tags and read as "not yet published".
+  uint64_t excl = 0;
+  if (tile > 0) {
+    int64_t j = static_cast<int64_t>(tile) - 1;
+    uint32_t spins = 0;
+    while (j >= 0) {
+      const uint64_t s = ld_acquire(status + static_cast<uint64_t>(j) * kRadix + t);
+      const uint64_t tag = s & ~kCountMask;
+      if (tag != tag_agg && tag != tag_pre) {
+        if (++spins > kSpinLimit) {
+          atomicOr(&prm->error, static_cast<int32_t>(kErrSortSpin));
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += s & kCountMask;
+      if (tag == tag_pre) break;
+      --j;
+    }
+    st_release(my_status, tag_pre | (excl + dtot));
+  }
+  s_goff[t] = goff + static_cast<uint32_t>(excl) - dstart;
+  __syncthreads();
+
+  // Reorder the tile in LDS by digit, then write runs of equal digits to
+  // consecutive global positions.
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    if (wbase + k * 64 + lane < n) {
+      const uint32_t d = dg[k];
+      const uint32_t li = s_dstart[d] + s_wcnt[wave][d] + rk[k];
+      s_x[li] = x[k]; s_y[li] = y[k]; s_i[li] = id[k]; s_d[li] = static_cast<uint8_t>(d);
+    }
+  }
+  __syncthreads();
+  const int tile_n = static_cast<int>(min(static_cast<int64_t>(kSortTile), n - base));
+  for (int j = t; j < tile_n; j += kSortThreads) {
+    const uint32_t pos = s_goff[s_d[j]] + static_cast<uint32_t>(j);
+    if (pos >= static_cast<uint64_t>(n)) {
+      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+      continue;
+    }
+    out_x[pos] = s_x[j];
+    out_y[pos] = s_y[j];
+    out_i[pos] = s_i[j];
+  }
+}
+
+// Cell runs over the sorted points (dbscan_grid.hpp:206-227): flags where the
+// key changes, block scan plus look-back, then point->cell, cell starts and
+// cell keys.  Blocked layout: thread t owns kScanItems consecutive points.
+__global__ __launch_bounds__(kScanThreads) void k_cells(
+    const float* __restrict__ xs_a, const float* __restrict__ ys_a,
+    const float* __restrict__ xs_b, const float* __restrict__ ys_b, Params* __restrict__ prm,
+    uint64_t* __restrict__ status, int32_t* __restrict__ point_cell,
+    int32_t* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
+    uint32_t* __restrict__ cell_flags, int64_t n, float inv_r) {
+  __shared__ uint64_t s_last[kScanThreads];
+  __shared__ uint32_t s_scan[kScanThreads / 64];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const KeyMap m = load_keymap(prm);
+  const bool in_a = (m.npasses & 1) != 0;
+  const float* __restrict__ xs = in_a ? xs_a : xs_b;
+  const float* __restrict__ ys = in_a ? ys_a : ys_b;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = atomicAdd(&prm->tile_counter[kCellTileCounter], 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int64_t base = static_cast<int64_t>(tile) * kScanTile;
+  const int64_t e0 = base + static_cast<int64_t>(t) * kScanItems;
+
+  uint64_t key[kScanItems];
+  uint32_t flags = 0, cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    key[j] = 0;
+    if (e0 + j < n) {
+      uint32_t xr, yr;
+      rel_cell(xs[e0 + j], ys[e0 + j], inv_r, m, xr, yr);
+      key[j] = cell_key64(xr, yr);
+    }
+  }
+  s_last[t] = key[kScanItems - 1];
+  __syncthreads();
+  uint64_t prev = 0;
+  bool have_prev = false;
+  if (t > 0) {
+    prev = s_last[t - 1];
+    have_prev = true;
+  } else if (base > 0) {
+    uint32_t xr, yr;
+    rel_cell(xs[base - 1], ys[base - 1], inv_r, m, xr, yr);
+    prev = cell_key64(xr, yr);
+    have_prev = true;
+  }
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (e0 + j < n) {
+      const bool f = !have_prev || key[j] != prev;
+      flags |= static_cast<uint32_t>(f) << j;
+      cnt += f;
+      prev = key[j];
+      have_prev = true;
+    }
+  }
+  uint32_t ttot;
+  const uint32_t texcl = block_excl_scan<kScanThreads>(cnt, ttot, s_scan);
+  if (t == 0) s_prefix = lookback_single(status, tile, ttot, &prm->error);
+  __syncthreads();
+  int64_t c = static_cast<int64_t>(s_prefix) + texcl - 1;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int64_t e = e0 + j;
+    if (e < n) {
+      if (c + ((flags >> j) & 1u) >= n) {
+        atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+        break;
+      }
+      if ((flags >> j) & 1u) {
+        ++c;
+        cell_start[c] = static_cast<int32_t>(e);
+        cell_key[c] = key[j];
+        cell_flags[c] = 0u;
+      }
+      point_cell[e] = static_cast<int32_t>(c);
+      if (e == n - 1) {
+        prm->num_cells = static_cast<int32_t>(c + 1);
+        cell_start[c + 1] = static_cast<int32_t>(n);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int32_t lower_bound_u64(const uint64_t* __restrict__ a, int32_t cnt,
+                                                   uint64_t v) {
+  int32_t lo = 0, hi = cnt;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Window columns, reference index, union-find init.  For column x' + dx the
+// cells with y' in [y'-2, y'+2] are a contiguous run [j0, j1) of cells, so
+// the 25-cell window of the reference (dbscan_grid.hpp:239-252) is 5 runs,
+// and their order (dx-major, then ky ascending) is the reference's window
+// offset order off = (dx+2)*5 + (dy+2).
+__global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
+                                             const uint64_t* __restrict__ cell_key,
+                                             int32_t* __restrict__ ngh,
+                                             int32_t* __restrict__ cell_ref,
+                                             int32_t* __restrict__ uf) {
+  const int32_t C = prm->num_cells;
+  const bool straddle = prm->kymin < 0 && prm->kymax >= 0;
+  const uint32_t yr0 = 0u - static_cast<uint32_t>(prm->kymin);  // y' of ky == 0
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
+       c += static_cast<int64_t>(gridDim.x) * 256) {
+    const uint64_t key = cell_key[c];
+    const uint32_t xr = static_cast<uint32_t>(key >> 32), yr = static_cast<uint32_t>(key);
+    const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
+    const uint32_t yhi = yr <= 0xFFFFFFFDu ? yr + 2u : 0xFFFFFFFFu;
+    int32_t* row = ngh + c * 10;
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      const int64_t cx = static_cast<int64_t>(xr) + dx;
+      int32_t j0 = 0, j1 = 0;
+      if (cx >= 0 && cx <= 0xFFFFFFFFll) {
+        const uint64_t colbase = static_cast<uint64_t>(cx) << 32;
+        j0 = lower_bound_u64(cell_key, C, colbase | ylo);
+        j1 = j0;
+        while (j1 < C && cell_key[j1] <= (colbase | yhi)) ++j1;
+      }
+      row[2 * (dx + 2)] = j0;
+      row[2 * (dx + 2) + 1] = j1;
+    }
+    int32_t r = static_cast<int32_t>(c);
+    if (straddle) {
+      const uint64_t colbase = static_cast<uint64_t>(xr) << 32;
+      const int32_t col_start = lower_bound_u64(cell_key, C, colbase);
+      const int32_t col_end =
+          xr == 0xFFFFFFFFu ? C : lower_bound_u64(cell_key, C, static_cast<uint64_t>(xr + 1u) << 32);
+      const int32_t split = lower_bound_u64(cell_key, C, colbase | yr0);
+      r = (yr >= yr0) ? col_start + (static_cast<int32_t>(c) - split)
+                      : static_cast<int32_t>(c) + (col_end - split);
+    }
+    if (r < 0 || r >= C) {
+      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+      r = static_cast<int32_t>(c);
+    }
+    cell_ref[c] = r;
+    uf[r] = r;
+  }
+}
+
+// Core marking (dbscan_grid.hpp:267-321).  A point in a cell of at least
+// min_samples points is core; otherwise its own cell counts wholesale and the
+// other window cells count the points with fl(fl(ex*ex) + fl(ey*ey)) <= eps_sq.
+__global__ __launch_bounds__(256) void k_core(
+    const Params* __restrict__ prm, const float* __restrict__ xs_a,
+    const float* __restrict__ ys_a, const float* __restrict__ xs_b,
+    const float* __restrict__ ys_b, const int32_t* __restrict__ point_cell,
+    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
+    uint8_t* __restrict__ is_core, uint32_t* __restrict__ cell_flags, int64_t n,
+    int32_t min_samples, float eps_sq) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const bool in_a = (prm->npasses & 1) != 0;
+  const float* __restrict__ xs = in_a ? xs_a : xs_b;
+  const float* __restrict__ ys = in_a ? ys_a : ys_b;
+  const int32_t c = point_cell[i];
+  const int32_t s = cell_start[c], e = cell_start[c + 1];
+  int32_t cnt = e - s;
+  bool core;
+  if (cnt >= min_samples) {
+    core = true;
+    if (i == s) cell_flags[c] = 1u;
+  } else {
+    const float xi = xs[i], yi = ys[i];
+    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+    for (int col = 0; col < 5 && cnt < min_samples; ++col) {
+      const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
+      if (j0 >= j1) continue;
+      const int32_t p0 = cell_start[j0], p1 = cell_start[j1];
+      for (int32_t j = p0; j < p1; ++j) {
+        if (j == s) {  // own cell: counted wholesale above
+          j = e - 1;
+          continue;
+        }
+        const float ex = xs[j] - xi, ey = ys[j] - yi;
+        if (ex * ex + ey * ey <= eps_sq) {
+          if (++cnt >= min_samples) break;
+        }
+      }
+    }
+    core = cnt >= min_samples;
+    atomicOr(cell_flags + c, core ? 1u : 2u);
+  }
+  is_core[i] = core ? 1 : 0;
+}
+
+// Cell-level union over eps-close core pairs (dbscan_grid.hpp:351-387).  Each
+// unordered window pair is visited once, from its smaller internal index:
+// the rest of the own column above c, and the two columns to the right.
+__global__ __launch_bounds__(256) void k_union(
+    Params* __restrict__ prm, const float* __restrict__ xs_a, const float* __restrict__ ys_a,
+    const float* __restrict__ xs_b, const float* __restrict__ ys_b,
+    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
+    const int32_t* __restrict__ cell_ref, const uint32_t* __restrict__ cell_flags,
+    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, float eps_sq) {
+  const int32_t C = prm->num_cells;
+  const bool in_a = (prm->npasses & 1) != 0;
+  const float* __restrict__ xs = in_a ? xs_a : xs_b;
+  const float* __restrict__ ys = in_a ? ys_a : ys_b;
+  for (int64_t c64 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c64 < C;
+       c64 += static_cast<int64_t>(gridDim.x) * 256) {
+    const int32_t c = static_cast<int32_t>(c64);
+    if (!(cell_flags[c] & 1u)) continue;
+    const int32_t rc = cell_ref[c];
+    const int32_t s_c = cell_start[c], e_c = cell_start[c + 1];
+    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+    for (int col = 2; col < 5; ++col) {
+      const int32_t j0 = (col == 2) ? c + 1 : row[2 * col];
+      const int32_t j1 = row[2 * col + 1];
+      for (int32_t nc = j0; nc < j1; ++nc) {
+        if (!(cell_flags[nc] & 1u)) continue;
+        const int32_t rn = cell_ref[nc];
+        if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
+        const int32_t s_n = cell_start[nc], e_n = cell_start[nc + 1];
+        bool found = false;
+        for (int32_t i = s_c; i < e_c && !found; ++i) {
+          if (!is_core[i]) continue;
+          const float xi = xs[i], yi = ys[i];
+          for (int32_t j = s_n; j < e_n; ++j) {
+            if (!is_core[j]) continue;
+            const float ex = xs[j] - xi, ey = ys[j] - yi;
+            if (ex * ex + ey * ey <= eps_sq) {
+              found = true;
+              break;
+            }
+          }
+        }
+        if (found) uf_unite(uf, rc, rn, &prm->error);
+      }
+    }
+  }
+}
+
+// uf[r] = root by a read-only walk (each thread writes only its own slot; the
+// reference's determinism fix, csrc/cuda/dbscan_cuda.cu:288-299,372-378), and
+// the root flag of each reference index.
+__global__ __launch_bounds__(256) void k_flatten(const Params* __restrict__ prm,
+                                                 const int32_t* __restrict__ cell_ref,
+                                                 const uint32_t* __restrict__ cell_flags,
+                                                 int32_t* __restrict__ uf,
+                                                 uint32_t* __restrict__ root_flag) {
+  const int32_t C = prm->num_cells;
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
+       c += static_cast<int64_t>(gridDim.x) * 256) {
+    const int32_t r = cell_ref[c];
+    const int32_t root = uf_find_ro(uf, r);
+    root_flag[r] = (root == r && (cell_flags[c] & 1u)) ? 1u : 0u;
+    st_release(uf + r, root);
+  }
+}
+
+// Exclusive scan of the root flags over reference indices, in place: the
+// rank of a root is its cluster id (dbscan_grid.hpp:392-398).
+__global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
+                                                       uint64_t* __restrict__ status,
+                                                       uint32_t* __restrict__ flag_rank) {
+  __shared__ uint32_t s_scan[kScanThreads / 64];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const int64_t C = prm->num_cells;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = atomicAdd(&prm->tile_counter[kRankTileCounter], 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const int64_t base = static_cast<int64_t>(tile) * kScanTile;
+  if (base >= C) return;  // no later tile waits on this one
+  const int64_t e0 = base + static_cast<int64_t>(t) * kScanItems;
+  uint32_t f[kScanItems];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    f[j] = (e0 + j < C) ? flag_rank[e0 + j] : 0u;
+    cnt += f[j];
+  }
+  uint32_t ttot;
+  const uint32_t texcl = block_excl_scan<kScanThreads>(cnt, ttot, s_scan);
+  if (t == 0) s_prefix = lookback_single(status, tile, ttot, &prm->error);
+  __syncthreads();
+  uint32_t run = static_cast<uint32_t>(s_prefix) + texcl;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (e0 + j < C) flag_rank[e0 + j] = run;
+    run += f[j];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
+                                                 const int32_t* __restrict__ cell_ref,
+                                                 const uint32_t* __restrict__ cell_flags,
+                                                 const int32_t* __restrict__ uf,
+                                                 const uint32_t* __restrict__ rank,
+                                                 int32_t* __restrict__ cell_cluster) {
+  const int32_t C = prm->num_cells;
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
+       c += static_cast<int64_t>(gridDim.x) * 256) {
+    cell_cluster[c] = (cell_flags[c] & 1u) ? static_cast<int32_t>(rank[uf[cell_ref[c]]]) : kNoise;
+  }
+}
+
+// Core labels, border assignment and unpermute (dbscan_grid.hpp:401-455).
+// A non-core point takes the label of the first core j, in window-offset
+// order, with d2 <= eps_sq and d2 strictly below the best so far (initially
+// eps_sq + 1.0f), i.e. the lexicographic minimum of (d2, off).
+__global__ __launch_bounds__(256) void k_label(
+    const Params* __restrict__ prm, const float* __restrict__ xs_a,
+    const float* __restrict__ ys_a, const uint32_t* __restrict__ idx_a,
+    const float* __restrict__ xs_b, const float* __restrict__ ys_b,
+    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
+    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
+    const uint32_t* __restrict__ cell_flags, const uint8_t* __restrict__ is_core,
+    const int32_t* __restrict__ cell_cluster, int32_t* __restrict__ labels, int64_t n,
+    float eps_sq, float best_init) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const bool in_a = (prm->npasses & 1) != 0;
+  const float* __restrict__ xs = in_a ? xs_a : xs_b;
+  const float* __restrict__ ys = in_a ? ys_a : ys_b;
+  const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
+  const int32_t c = point_cell[i];
+  int32_t lab;
+  if (is_core[i]) {
+    lab = cell_cluster[c];
+  } else {
+    lab = kNoise;
+    float best = best_init;
+    const float xi = xs[i], yi = ys[i];
+    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+    for (int col = 0; col < 5; ++col) {
+      const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
+      for (int32_t nc = j0; nc < j1; ++nc) {
+        if (!(cell_flags[nc] & 1u)) continue;
+        const int32_t p1 = cell_start[nc + 1];
+        for (int32_t j = cell_start[nc]; j < p1; ++j) {
+          if (!is_core[j]) continue;
+          const float ex = xs[j] - xi, ey = ys[j] - yi;
+          const float d2 = ex * ex + ey * ey;
+          if (d2 <= eps_sq && d2 < best) {
+            best = d2;
+            lab = cell_cluster[nc];
+          }
+        }
+      }
+    }
+  }
+  const uint32_t dst = idx[i];
+  if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct Layout {
+  size_t params, sort_status, cell_status, rank_status, zero_end;
+  size_t partials;
+  size_t xs_a, ys_a, idx_a, xs_b, ys_b, idx_b;
+  size_t point_cell, is_core, cell_start, cell_key, cell_flags, ngh, cell_ref, uf, rank,
+      cell_cluster;
+  size_t total;
+};
+
+constexpr size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+Layout make_layout(int64_t n) {
+  const size_t N = static_cast<size_t>(n > 0 ? n : 1);
+  const size_t sort_tiles = (N + kSortTile - 1) / kSortTile;
+  const size_t scan_tiles = (N + kScanTile - 1) / kScanTile;
+  Layout L{};
+  size_t off = 0;
+  auto take = [&off](size_t bytes) {
+    const size_t at = off;
+    off = align256(off + bytes);
+    return at;
+  };
+  L.params = take(sizeof(Params));
+  L.sort_status = take(sort_tiles * kRadix * sizeof(uint64_t));
+  L.cell_status = take(scan_tiles * sizeof(uint64_t));
+  L.rank_status = take(scan_tiles * sizeof(uint64_t));
+  L.zero_end = off;
+  L.partials = take(kMaxReduceBlocks * sizeof(int4));
+  L.xs_a = take(N * 4);
+  L.ys_a = take(N * 4);
+  L.idx_a = take(N * 4);
+  L.xs_b = take(N * 4);
+  L.ys_b = take(N * 4);
+  L.idx_b = take(N * 4);
+  L.point_cell = take(N * 4);
+  L.is_core = take(N);
+  L.cell_start = take((N + 1) * 4);
+  L.cell_key = take(N * 8);
+  L.cell_flags = take(N * 4);
+  L.ngh = take(N * 10 * 4);
+  L.cell_ref = take(N * 4);
+  L.uf = take(N * 4);
+  L.rank = take(N * 4);
+  L.cell_cluster = take(N * 4);
+  L.total = off;
+  return L;
+}
+
+thread_local int g_last_hip_error = 0;
+
+const char* const kStageNames[DBSCAN_HIP_NUM_STAGES] = {
+    "cell_keys", "sort_gather", "boundary_scan", "build_ngh", "core_mark",
+    "union_find", "cluster_id", "core_labels", "border_assign", "unpermute"};
+
+int grid_for(int64_t items, int per_block, int cap) {
+  int64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<int>(g);
+}
+
+template <typename T>
+T* at(void* ws, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
+}
+
+int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
+        void* workspace, size_t workspace_bytes, hipStream_t stream, float* stage_ms) {
+  if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
+  if (!(eps > 0.0)) return DBSCAN_HIP_ERR_EPS;  // csrc/extension.cpp:29
+  if (min_samples64 < 1) return DBSCAN_HIP_ERR_MIN_SAMPLES;  // csrc/extension.cpp:30
+  if (stage_ms)
+    for (int s = 0; s < DBSCAN_HIP_NUM_STAGES; ++s) stage_ms[s] = 0.f;
+  if (n == 0) return DBSCAN_HIP_OK;
+  if (!X || !labels || !workspace) return DBSCAN_HIP_ERR_NULL;
+  if ((reinterpret_cast<uintptr_t>(X) & 7u) || (reinterpret_cast<uintptr_t>(labels) & 3u) ||
+      (reinterpret_cast<uintptr_t>(workspace) & 255u))
+    return DBSCAN_HIP_ERR_ALIGN;
+  const Layout L = make_layout(n);
+  if (workspace_bytes < L.total) return DBSCAN_HIP_ERR_WORKSPACE;
+
+  // Scalar parameters exactly as the reference computes them
+  // (dbscan_cpu.cpp:52-53, dbscan_grid.hpp:153-154,425).
+  const float eps_f = static_cast<float>(eps);
+  const int32_t ms = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(min_samples64)));
+  const float inv_r = static_cast<float>(std::sqrt(2.0)) / eps_f;
+  const float eps_sq = eps_f * eps_f;
+  const float best_init = eps_sq + 1.0f;
+
+  hipEvent_t ev[DBSCAN_HIP_NUM_STAGES + 1] = {};
+  auto fail = [&](hipError_t e) {
+    g_last_hip_error = static_cast<int>(e);
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
+    return DBSCAN_HIP_ERR_HIP;
+  };
+  if (stage_ms) {
+    for (auto& x : ev) {
+      const hipError_t e = hipEventCreate(&x);
+      if (e != hipSuccess) return fail(e);
+    }
+  }
+  auto mark = [&](int s) -> hipError_t {
+    return stage_ms ? hipEventRecord(ev[s], stream) : hipSuccess;
+  };
+
+  if (ms < 1) {  // dbscan_grid.hpp:141: every point is noise
+    k_fill_i32<<<grid_for(n, 256, 4096), 256, 0, stream>>>(labels, n, kNoise);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DBSCAN_HIP_OK : fail(e);
+  }
+
+  void* ws = workspace;
+  Params* prm = at<Params>(ws, L.params);
+  uint64_t* sort_status = at<uint64_t>(ws, L.sort_status);
+  const float2* X2 = reinterpret_cast<const float2*>(X);
+  float* xs_a = at<float>(ws, L.xs_a);
+  float* ys_a = at<float>(ws, L.ys_a);
+  uint32_t* idx_a = at<uint32_t>(ws, L.idx_a);
+  float* xs_b = at<float>(ws, L.xs_b);
+  float* ys_b = at<float>(ws, L.ys_b);
+  uint32_t* idx_b = at<uint32_t>(ws, L.idx_b);
+  int32_t* point_cell = at<int32_t>(ws, L.point_cell);
+  uint8_t* is_core = at<uint8_t>(ws, L.is_core);
+  int32_t* cell_start = at<int32_t>(ws, L.cell_start);
+  uint64_t* cell_key = at<uint64_t>(ws, L.cell_key);
+  uint32_t* cell_flags = at<uint32_t>(ws, L.cell_flags);
+  int32_t* ngh = at<int32_t>(ws, L.ngh);
+  int32_t* cell_ref = at<int32_t>(ws, L.cell_ref);
+  int32_t* uf = at<int32_t>(ws, L.uf);
+  uint32_t* rank = at<uint32_t>(ws, L.rank);
+  int32_t* cell_cluster = at<int32_t>(ws, L.cell_cluster);
+
+  const int g_reduce = grid_for(n, 2048, kMaxReduceBlocks);
+  const int g_hist = grid_for(n, 4096, kMaxReduceBlocks);
+  const int g_sort = grid_for(n, kSortTile, INT_MAX);
+  const int g_scan = grid_for(n, kScanTile, INT_MAX);
+  const int g_point = grid_for(n, 256, INT_MAX);
+  const int g_cells = grid_for(n, 256, 16384);
+  hipError_t e;
+
+  // Stage 1: cell keys (bounds + digit histograms).
+  if ((e = mark(0)) != hipSuccess) return fail(e);
+  const int64_t zero16 = static_cast<int64_t>((L.zero_end - kParamsZeroBegin) / 16);
+  k_bounds<<<g_reduce, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials),
+                                         at<uint4>(ws, kParamsZeroBegin), zero16);
+  k_hist<<<g_hist, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm);
+
+  // Stage 2: radix passes (sort + gather in one).
+  if ((e = mark(1)) != hipSuccess) return fail(e);
+  k_sort_pass<true><<<g_sort, kSortThreads, 0, stream>>>(X2, nullptr, nullptr, nullptr, xs_a,
+                                                         ys_a, idx_a, prm, sort_status, 0, n,
+                                                         inv_r);
+  for (int p = 1; p < kMaxPasses; ++p) {
+    const bool from_a = (p & 1) != 0;
+    k_sort_pass<false><<<g_sort, kSortThreads, 0, stream>>>(
+        nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
+        from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, sort_status, p,
+        n, inv_r);
+  }
+
+  // Stage 3: cell runs.
+  if ((e = mark(2)) != hipSuccess) return fail(e);
+  k_cells<<<g_scan, kScanThreads, 0, stream>>>(xs_a, ys_a, xs_b, ys_b, prm,
+                                               at<uint64_t>(ws, L.cell_status), point_cell,
+                                               cell_start, cell_key, cell_flags, n, inv_r);
+
+  // Stage 4: window columns.
+  if ((e = mark(3)) != hipSuccess) return fail(e);
+  k_ngh<<<g_cells, 256, 0, stream>>>(prm, cell_key, ngh, cell_ref, uf);
+
+  // Stage 5: core marking.
+  if ((e = mark(4)) != hipSuccess) return fail(e);
+  k_core<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start, ngh,
+                                      is_core, cell_flags, n, ms, eps_sq);
+
+  // Stage 6: union-find.
+  if ((e = mark(5)) != hipSuccess) return fail(e);
+  k_union<<<g_cells, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
+                                       cell_flags, is_core, uf, eps_sq);
+  k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
+
+  // Stage 7: cluster ids.
+  if ((e = mark(6)) != hipSuccess) return fail(e);
+  k_rank<<<g_scan, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank);
+  k_cluster<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank, cell_cluster);
+
+  // Stages 8-10: core labels, border assignment and unpermute, fused.
+  if ((e = mark(7)) != hipSuccess) return fail(e);
+  if ((e = mark(8)) != hipSuccess) return fail(e);
+  k_label<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, point_cell,
+                                       cell_start, ngh, cell_flags, is_core, cell_cluster, labels,
+                                       n, eps_sq, best_init);
+  if ((e = mark(9)) != hipSuccess) return fail(e);
+  if ((e = mark(10)) != hipSuccess) return fail(e);
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+
+  if (stage_ms) {
+    if ((e = hipEventSynchronize(ev[DBSCAN_HIP_NUM_STAGES])) != hipSuccess) return fail(e);
+    for (int s = 0; s < DBSCAN_HIP_NUM_STAGES; ++s) {
+      if ((e = hipEventElapsedTime(&stage_ms[s], ev[s], ev[s + 1])) != hipSuccess) return fail(e);
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+  }
+  return DBSCAN_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dbscan2d_hip_workspace_bytes(int64_t n, size_t* bytes) {
+  if (!bytes) return DBSCAN_HIP_ERR_NULL;
+  if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
+  *bytes = make_layout(n).total;
+  return DBSCAN_HIP_OK;
+}
+
+int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples, int32_t* labels,
+                 void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr);
+}
+
+int dbscan2d_hip_profile(const float* X, int64_t n, double eps, int64_t min_samples,
+                         int32_t* labels, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream, float* stage_ms) {
+  if (!stage_ms) return DBSCAN_HIP_ERR_NULL;
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, stage_ms);
+}
+
+const char* dbscan2d_hip_stage_name(int i) {
+  return (i >= 0 && i < DBSCAN_HIP_NUM_STAGES) ? kStageNames[i] : nullptr;
+}
+
+int dbscan2d_hip_layout(int64_t n, dbscan_hip_layout* out) {
+  if (!out) return DBSCAN_HIP_ERR_NULL;
+  if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
+  const Layout L = make_layout(n);
+  out->info = L.params;
+  out->xs_a = L.xs_a; out->ys_a = L.ys_a; out->idx_a = L.idx_a;
+  out->xs_b = L.xs_b; out->ys_b = L.ys_b; out->idx_b = L.idx_b;
+  out->point_cell = L.point_cell;
+  out->is_core = L.is_core;
+  out->cell_start = L.cell_start;
+  out->cell_key = L.cell_key;
+  out->cell_flags = L.cell_flags;
+  out->cell_ref = L.cell_ref;
+  out->uf = L.uf;
+  out->cell_cluster = L.cell_cluster;
+  return DBSCAN_HIP_OK;
+}
+
+const char* dbscan2d_hip_status_string(int status) {
+  switch (status) {
+    case DBSCAN_HIP_OK: return "ok";
+    case DBSCAN_HIP_ERR_N: return "n must be in [0, 2^31)";
+    case DBSCAN_HIP_ERR_EPS: return "eps must be > 0";
+    case DBSCAN_HIP_ERR_MIN_SAMPLES: return "min_samples must be >= 1";
+    case DBSCAN_HIP_ERR_NULL: return "null pointer argument";
+    case DBSCAN_HIP_ERR_WORKSPACE: return "workspace too small";
+    case DBSCAN_HIP_ERR_ALIGN: return "misaligned pointer (X 8 B, labels 4 B, workspace 256 B)";
+    case DBSCAN_HIP_ERR_HIP: return "HIP runtime error";
+    default: return "unknown status";
+  }
+}
+
+int dbscan2d_hip_last_hip_error(void) { return g_last_hip_error; }
+
+}  // extern "C"

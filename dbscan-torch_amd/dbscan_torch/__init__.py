@@ -1,0 +1,104 @@
+"""dbscan_torch on MI355X: 2-D grid-hashed DBSCAN for PyTorch-ROCm.
+
+Drop-in for the reference package's public surface
+(dbscan_torch/__init__.py:45-118): ``dbscan2d``, ``DBSCAN``, ``ArrayLike2D``
+and ``__version__``, with the same argument meaning, input/output types and
+error behaviour.  The clustering runs in hand-written gfx950 HIP kernels
+(libdbscan_hip.so) reached through the pybind module ``_C``; labels are
+bit-identical to the reference's CPU path.
+"""
+
+from importlib.metadata import PackageNotFoundError
+from importlib.metadata import version as _pkg_version
+from typing import Any, Optional, Union
+
+import numpy as np
+import torch
+
+# Same guard as the reference (dbscan_torch/__init__.py:11-38): a binding built
+# against another torch minor version fails to load with an obscure symbol
+# error, so name the mismatch first.
+try:
+    from ._build_info import BUILD_TORCH_VERSION as _build_torch
+except ImportError:
+    _build_torch = None
+
+if _build_torch is not None:
+    try:
+        _mismatch = tuple(int(x) for x in _build_torch.split(".")[:2]) != tuple(
+            int(x) for x in torch.__version__.split(".")[:2])
+    except (ValueError, IndexError, AttributeError):
+        _mismatch = False
+    if _mismatch:
+        raise ImportError(
+            f"dbscan_torch was built against torch {_build_torch} but the installed runtime "
+            f"torch is {torch.__version__}. Rebuild with: python dbscan-torch_amd/build.py")
+
+try:
+    from . import _C
+except ImportError as exc:  # the native library is the product: fail loudly
+    raise ImportError(
+        "dbscan_torch: the native extension (_C, libdbscan_hip.so) is not built or does not "
+        "load. Build it with: python dbscan-torch_amd/build.py") from exc
+
+ArrayLike2D = Union[np.ndarray, torch.Tensor]
+
+__all__ = ["dbscan2d", "DBSCAN", "ArrayLike2D"]
+
+try:
+    __version__ = _pkg_version("dbscan-torch")
+except PackageNotFoundError:
+    __version__ = "0.2.0+mi355x"
+
+
+def dbscan2d(X: torch.Tensor, eps: float, min_samples: int) -> torch.Tensor:
+    """Run DBSCAN on a 2D point set.
+
+    The output tensor is on the same device as the input.  ROCm tensors are
+    clustered in place on their device with no host round trip; host tensors
+    are staged to the current HIP device and the labels copied back.
+
+    Args:
+        X: float32 tensor of shape (N, 2).
+        eps: neighborhood radius (> 0).
+        min_samples: minimum neighbors (incl. self) to be a core point (>= 1).
+
+    Returns:
+        int32 tensor of shape (N,). -1 indicates noise, otherwise a cluster id
+        in ``[0, n_clusters)``, numbered as the reference CPU path numbers them.
+    """
+    return _C.dbscan2d(X, float(eps), int(min_samples))
+
+
+class DBSCAN:
+    """sklearn-compatible class wrapper around :func:`dbscan2d`
+    (reference dbscan_torch/__init__.py:76-118).
+
+    Input type is preserved end-to-end: ``numpy.ndarray`` in ->
+    ``numpy.ndarray`` out; ``torch.Tensor`` in -> ``torch.Tensor`` out on the
+    same device.  Non-float32 input is cast to float32.
+
+    Args:
+        eps: neighborhood radius (> 0).
+        min_samples: minimum neighbors (incl. self) to be a core point (>= 1).
+    """
+
+    def __init__(self, eps: float = 0.5, min_samples: int = 5):
+        self.eps = float(eps)
+        self.min_samples = int(min_samples)
+        self.labels_: Optional[ArrayLike2D] = None
+
+    def fit(self, X: ArrayLike2D, y: Any = None) -> "DBSCAN":
+        """Fit the clusterer on ``X``; labels go to ``self.labels_``.  ``y`` is ignored."""
+        is_torch = isinstance(X, torch.Tensor)
+        if is_torch:
+            Xt = X if X.dtype == torch.float32 else X.to(torch.float32)
+        else:
+            Xt = torch.as_tensor(X, dtype=torch.float32)
+        labels = dbscan2d(Xt, self.eps, self.min_samples)
+        self.labels_ = labels if is_torch else labels.cpu().numpy()
+        return self
+
+    def fit_predict(self, X: ArrayLike2D, y: Any = None) -> ArrayLike2D:
+        """Fit and return labels in one call."""
+        return self.fit(X, y).labels_  # type: ignore[return-value]

@@ -1,0 +1,116 @@
+/*
+ * dbscan_hip.h -- C-ABI of the MI355X (gfx950) 2-D grid DBSCAN.
+ *
+ * Exported by libdbscan_hip.so.  Plain pointers, sizes and a HIP stream; no
+ * torch types.  The library allocates nothing: the caller passes device
+ * memory for the input, the labels and one workspace buffer, and all work is
+ * enqueued on `stream` without a host synchronisation (except in the profile
+ * entry point, which waits for its own timing events).
+ *
+ * Semantics are those of the reference CPU path (csrc/cpu/dbscan_grid.hpp
+ * :136-458): labels are bit-identical to it, -1 marks noise, and cluster ids
+ * are numbered in the reference's cell order.
+ */
+#ifndef DBSCAN_HIP_H_
+#define DBSCAN_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by every entry point. */
+enum {
+  DBSCAN_HIP_OK = 0,
+  DBSCAN_HIP_ERR_N = 1,              /* n < 0 or n > INT32_MAX                     */
+  DBSCAN_HIP_ERR_EPS = 2,            /* eps is not > 0 (NaN included)              */
+  DBSCAN_HIP_ERR_MIN_SAMPLES = 3,    /* min_samples < 1                            */
+  DBSCAN_HIP_ERR_NULL = 4,           /* a required pointer is NULL                 */
+  DBSCAN_HIP_ERR_WORKSPACE = 5,      /* workspace smaller than the required bytes  */
+  DBSCAN_HIP_ERR_ALIGN = 6,          /* X, labels or workspace misaligned          */
+  DBSCAN_HIP_ERR_HIP = 7             /* a HIP runtime call failed (see last error) */
+};
+
+/* Number of stages reported by dbscan2d_hip_profile, and their names.  The
+ * names are the reference's kStageNames (csrc/cuda/dbscan_cuda.cu:513-524,
+ * csrc/cpu/dbscan_grid.hpp:67-78). */
+#define DBSCAN_HIP_NUM_STAGES 10
+
+/*
+ * Workspace bytes needed for n points.  Pure host arithmetic; no device call.
+ * Replaces the per-call torch::empty allocations of the reference's
+ * dbscan2d_cuda_impl (csrc/cuda/dbscan_cuda.cu:573-732).
+ */
+int dbscan2d_hip_workspace_bytes(int64_t n, size_t* bytes);
+
+/*
+ * Clusters the n points X[2*i], X[2*i+1] (float32, interleaved, device
+ * memory, 8-byte aligned) and writes n int32 labels to device memory
+ * `labels`.  eps and min_samples are the reference's arguments: eps is
+ * validated as a double (> 0) and rounded to float, min_samples is validated
+ * as >= 1 and narrowed to int32 (csrc/extension.cpp:26-31,
+ * csrc/cpu/dbscan_cpu.cpp:52-53).  `workspace` (256-byte aligned) holds at
+ * least dbscan2d_hip_workspace_bytes(n) bytes.
+ *
+ * Replaces dbscan2d_cuda (csrc/cuda/dbscan_cuda.cu:752-755), which the
+ * reference's binding calls at csrc/extension.cpp:36-37.
+ */
+int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples,
+                 int32_t* labels, void* workspace, size_t workspace_bytes,
+                 hipStream_t stream);
+
+/*
+ * Same as dbscan2d_hip, and additionally writes the GPU time of each of the
+ * DBSCAN_HIP_NUM_STAGES stages, in milliseconds, to stage_ms (host memory).
+ * Synchronises on its own events before returning.
+ *
+ * Replaces dbscan2d_cuda_profile (csrc/cuda/dbscan_cuda.cu:760-770), bound at
+ * csrc/extension.cpp:52-63.
+ */
+int dbscan2d_hip_profile(const float* X, int64_t n, double eps, int64_t min_samples,
+                         int32_t* labels, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream, float* stage_ms);
+
+/* Name of stage i (0 <= i < DBSCAN_HIP_NUM_STAGES), or NULL. */
+const char* dbscan2d_hip_stage_name(int i);
+
+/*
+ * Byte offsets, inside the workspace of a dbscan2d_hip call on n points, of
+ * the intermediates that call leaves behind (all in the library's sorted
+ * point order and internal cell order).  For stage-wise comparison with the
+ * oracle; replaces the intermediates of dbscan2d_cuda_debug
+ * (csrc/cuda/dbscan_cuda.cu:777-800).  Fields are filled for `n` points; the
+ * counts that are only known on the device (cells, sort passes) are read from
+ * the `info` block.
+ */
+typedef struct {
+  size_t info;        /* int32[8]: kxmin, kxmax, kymin, kymax, num_cells, npasses, 0, 0 */
+  size_t xs_a, ys_a, idx_a;   /* sorted x, y, original index if npasses is odd  */
+  size_t xs_b, ys_b, idx_b;   /* ... if npasses is even                         */
+  size_t point_cell;  /* int32[n]  cell of each sorted point                      */
+  size_t is_core;     /* uint8[n]  core flag of each sorted point                 */
+  size_t cell_start;  /* int32[num_cells+1]                                       */
+  size_t cell_key;    /* uint64[num_cells]  (x' << 32 | y'), internal coordinates */
+  size_t cell_flags;  /* uint32[num_cells]  bit0 has core, bit1 has non-core      */
+  size_t cell_ref;    /* int32[num_cells]   reference-order index of each cell    */
+  size_t uf;          /* int32[num_cells]   union-find root, by reference index   */
+  size_t cell_cluster;/* int32[num_cells]   cluster id, by internal index, or -1  */
+} dbscan_hip_layout;
+
+int dbscan2d_hip_layout(int64_t n, dbscan_hip_layout* layout);
+
+/* Human-readable text for a status code. */
+const char* dbscan2d_hip_status_string(int status);
+
+/* The hipError_t behind the most recent DBSCAN_HIP_ERR_HIP on this thread. */
+int dbscan2d_hip_last_hip_error(void);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* DBSCAN_HIP_H_ */

@@ -1,0 +1,124 @@
+"""Host-side checks that need no GPU: the binding's validation (same order and
+messages as the reference, csrc/extension.cpp:26-31), the C-ABI library's
+exports and its argument checks, which return before any device call."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import dbscan_torch
+from dbscan_torch import DBSCAN, dbscan2d
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dbscan_hip.h")
+LIB = os.path.join(os.path.dirname(dbscan_torch.__file__), "libdbscan_hip.so")
+
+
+def test_input_validation_messages():
+    # tests/test_basic.py:67-75 of the reference, same patterns
+    with pytest.raises(RuntimeError, match=r"shape \(N, 2\)"):
+        dbscan2d(torch.zeros((10, 3)), eps=1.0, min_samples=5)
+    with pytest.raises(RuntimeError, match="float32"):
+        dbscan2d(torch.zeros((10, 2), dtype=torch.float64), eps=1.0, min_samples=5)
+    with pytest.raises(RuntimeError, match="eps"):
+        dbscan2d(torch.zeros((10, 2)), eps=0.0, min_samples=5)
+    with pytest.raises(RuntimeError, match="min_samples"):
+        dbscan2d(torch.zeros((10, 2)), eps=1.0, min_samples=0)
+
+
+def test_validation_order():
+    # shape is checked before dtype, dtype before eps, eps before min_samples
+    with pytest.raises(RuntimeError, match=r"shape \(N, 2\)"):
+        dbscan2d(torch.zeros((4, 3), dtype=torch.float64), eps=-1.0, min_samples=0)
+    with pytest.raises(RuntimeError, match="float32"):
+        dbscan2d(torch.zeros((4, 2), dtype=torch.float64), eps=-1.0, min_samples=0)
+    with pytest.raises(RuntimeError, match="eps"):
+        dbscan2d(torch.zeros((4, 2)), eps=float("nan"), min_samples=0)
+    with pytest.raises(RuntimeError, match="eps"):
+        dbscan2d(torch.zeros((4, 2)), eps=-1.0, min_samples=0)
+
+
+def test_empty_input_needs_no_device():
+    X = torch.empty((0, 2), dtype=torch.float32)
+    labels = dbscan2d(X, eps=1.0, min_samples=5)
+    assert labels.shape == (0,) and labels.dtype == torch.int32 and labels.device == X.device
+    labels, stages = dbscan_torch._C.dbscan2d_profile(X, 1.0, 5)
+    assert labels.shape == (0,)
+    assert list(stages) == sorted(["cell_keys", "sort_gather", "boundary_scan", "build_ngh",
+                                   "core_mark", "union_find", "cluster_id", "core_labels",
+                                   "border_assign", "unpermute"])
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-device error path")
+def test_host_tensor_without_device_raises():
+    X = torch.zeros((10, 2))
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        dbscan2d(X, eps=1.0, min_samples=5)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        DBSCAN(eps=1.0, min_samples=5).fit(np.zeros((10, 2), np.float32))
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    return re.findall(r"^(?:int|const char\*)\s+(dbscan2d_\w+)\s*\(", text, re.M)
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared_functions()
+    assert {"dbscan2d_hip", "dbscan2d_hip_workspace_bytes", "dbscan2d_hip_profile"} <= set(names)
+    lib = ctypes.CDLL(LIB)
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    lib.dbscan2d_hip.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.c_int64, P, P,
+                                 ctypes.c_size_t, P]
+    lib.dbscan2d_hip.restype = ctypes.c_int
+    lib.dbscan2d_hip_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]
+    lib.dbscan2d_hip_status_string.restype = ctypes.c_char_p
+    lib.dbscan2d_hip_stage_name.restype = ctypes.c_char_p
+    return lib
+
+
+def test_c_abi_argument_checks(lib):
+    # codes from include/dbscan_hip.h; each returns before any device work
+    assert lib.dbscan2d_hip(None, -1, 1.0, 5, None, None, 0, None) == 1
+    assert lib.dbscan2d_hip(None, 2**31, 1.0, 5, None, None, 0, None) == 1
+    assert lib.dbscan2d_hip(None, 10, 0.0, 5, None, None, 0, None) == 2
+    assert lib.dbscan2d_hip(None, 10, float("nan"), 5, None, None, 0, None) == 2
+    assert lib.dbscan2d_hip(None, 10, 1.0, 0, None, None, 0, None) == 3
+    assert lib.dbscan2d_hip(None, 0, 1.0, 5, None, None, 0, None) == 0
+    assert lib.dbscan2d_hip(None, 10, 1.0, 5, None, None, 0, None) == 4
+    # misaligned workspace, then a too-small one (fake addresses, never touched)
+    assert lib.dbscan2d_hip(0x1000, 10, 1.0, 5, 0x2000, 0x3001, 1 << 30, None) == 6
+    assert lib.dbscan2d_hip(0x1000, 10, 1.0, 5, 0x2000, 0x3000, 16, None) == 5
+    assert lib.dbscan2d_hip_status_string(5) == b"workspace too small"
+
+
+def test_workspace_and_layout(lib):
+    sizes = []
+    for n in (1, 1000, 4096, 4097, 10_000_000):
+        b = ctypes.c_size_t(0)
+        assert lib.dbscan2d_hip_workspace_bytes(n, ctypes.byref(b)) == 0
+        sizes.append(b.value)
+    assert sizes == sorted(sizes)
+    # about 110 bytes per point at scale
+    assert 100 * 10_000_000 < sizes[-1] < 130 * 10_000_000
+    names = [lib.dbscan2d_hip_stage_name(i).decode() for i in range(10)]
+    assert names[0] == "cell_keys" and names[-1] == "unpermute"
+    assert lib.dbscan2d_hip_stage_name(10) is None
+
+
+def test_class_wrapper_contract_without_compute():
+    clf = DBSCAN()
+    assert (clf.eps, clf.min_samples, clf.labels_) == (0.5, 5, None)
+    out = DBSCAN(eps=1.0, min_samples=3).fit(np.zeros((0, 2), np.float32))
+    assert isinstance(out.labels_, np.ndarray) and out.labels_.shape == (0,)
