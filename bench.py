@@ -1,0 +1,185 @@
+"""Benchmark of the gfx950 dbscan2d path on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3a]
+
+A step is one dbscan2d call on the resident N = 10,000,000-point
+constant-density workload (1,000 Gaussian blobs, std 0.5, centres in
+[0,1265]^2, plus 5% uniform noise in [-126.5,1391.5]^2, eps 0.5,
+min_samples 10, seed 2).  Prints one JSON line:
+
+  value          Mpts/s over K timed steps (input already in HBM)
+  roofline       the dominant kernel's algorithmic bytes per launch over its
+                 mean launch time, from HIP events on the launch stream inside
+                 the timed region, against the 8 TB/s HBM peak
+  e2e_roofline   the end-to-end compulsory-traffic model (107 B/point +
+                 40 B/cell) over the step time
+  cpu_baseline   the reference's own CPU path (compiled from its sources into
+                 oracle/_ref) on the same workload, on the host cores, or the C
+                 oracle where that build is absent
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "dbscan-torch_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch of each kernel, per point (N) and per cell (C):
+# what the kernel must read and write at minimum.
+#   k_sort_pass[0]  reads X (8) and writes x, y, index (12)
+#   k_sort_pass[p]  reads and writes x, y, index (12 + 12)
+#   k_label         reads point_cell (4), is_core (1), index (4), writes the
+#                   label (4); border points' window reads are not counted
+#   k_core          reads point_cell (4), writes is_core (1); cell_start 8/cell
+#   k_cells         reads x, y (8), writes point_cell (4); start+key 12/cell
+KERNEL_BYTES = {
+    "k_sort_pass[0]": (20, 0),
+    "k_sort_pass[1]": (24, 0), "k_sort_pass[2]": (24, 0), "k_sort_pass[3]": (24, 0),
+    "k_sort_pass[4]": (24, 0), "k_sort_pass[5]": (24, 0), "k_sort_pass[6]": (24, 0),
+    "k_sort_pass[7]": (24, 0),
+    "k_bounds": (8, 0), "k_hist": (8, 0),
+    "k_cells": (12, 12), "k_core": (5, 8), "k_label": (13, 4),
+    "k_ngh": (0, 48), "k_union": (9, 12), "k_flatten": (0, 12), "k_rank": (0, 8),
+    "k_cluster": (0, 16),
+}
+
+
+def kernel_groups(names, npasses):
+    """Launch slots grouped per kernel function (a sort pass beyond npasses
+    returns at once and is left out)."""
+    groups = {}
+    for i, name in enumerate(names):
+        if name.startswith("k_sort_pass["):
+            p = int(name[len("k_sort_pass["):-1])
+            if p >= npasses:
+                continue
+            key = "k_sort_pass<first>" if p == 0 else "k_sort_pass<rest>"
+        else:
+            key = name
+        groups.setdefault(key, []).append(i)
+    return groups
+
+
+def cpu_baseline(X, eps, ms, reps=3):
+    import numpy as np
+    import torch
+
+    import oracle
+
+    threads = torch.get_num_threads()
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    if oracle.ref_available():
+        kind, fn = "reference", oracle.ref_dbscan2d
+    else:
+        kind, fn = "port", oracle.dbscan2d
+        oracle.set_threads(threads)
+    fn(X, eps, ms)  # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn(X, eps, ms)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {
+        "value": round(len(X) / t / 1e6, 3), "unit": "Mpts/s", "cores": threads, "kind": kind,
+        "sample": f"full workload ({len(X)} points), median of {reps} calls after 1 warm-up, "
+                  f"torch.get_num_threads()={threads}, OMP_NUM_THREADS={os.environ['OMP_NUM_THREADS']}",
+        "seconds_per_call": round(t, 4),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cfg3a", choices=["cfg2", "cfg3a", "cfg3b"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the single-cloud path does not shard across GPUs)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import dbscan_torch
+    from tests import workloads as W
+
+    X, eps, ms = W.CONFIGS[args.workload]()
+    n = len(X)
+    Xd = torch.from_numpy(X).cuda()
+    for _ in range(args.warmup):
+        dbscan_torch.dbscan2d(Xd, eps, ms)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    labels, kernel_ms, names, event_total_ms = dbscan_torch._C.dbscan2d_benchmark(
+        Xd, eps, ms, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+
+    dbg = dbscan_torch._C.dbscan2d_hip_debug(Xd, eps, ms)
+    info = dbg[1].cpu().numpy()
+    num_cells, npasses, err = int(info[4]), int(info[5]), int(info[6])
+    del dbg
+
+    groups = kernel_groups(names, npasses)
+    per_kernel = {}
+    for key, slots in groups.items():
+        t = sum(kernel_ms[i] for i in slots)
+        b = sum(KERNEL_BYTES[names[i]][0] * n + KERNEL_BYTES[names[i]][1] * num_cells for i in slots)
+        per_kernel[key] = {"launches": len(slots), "ms_per_launch": t / len(slots),
+                           "gbs": b / (t * 1e-3) / 1e9 if t > 0 else 0.0,
+                           "bytes_per_launch": b / len(slots)}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_launch"] * per_kernel[k]["launches"])
+    d = per_kernel[dom]
+    e2e_bytes = 107 * n + 40 * num_cells
+    e2e_gbs = e2e_bytes / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "Mpts/s + HBM GB/s at N=10M (single-call dbscan2d)",
+        "value": round(n / (ms_per_step * 1e-3) / 1e6, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, resident in HBM before timing)",
+        "config": {"workload": {"cfg2": "cfg2 1.07M blobs+lattice eps0.5 ms10",
+                                "cfg3a": "cfg3a 10M constant-density eps0.5 ms10",
+                                "cfg3b": "cfg3b 10M skewed eps0.25 ms20"}[args.workload],
+                   "n_points": n, "eps": eps, "min_samples": ms, "num_cells": num_cells,
+                   "sort_passes": npasses},
+        "roofline": {"bound": "hbm", "kernel": dom,
+                     "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(d["bytes_per_launch"]),
+                     "ms_per_launch": round(d["ms_per_launch"], 4)},
+        "e2e_roofline": {"bytes_model": "107*N + 40*C", "bytes": e2e_bytes,
+                         "achieved": round(e2e_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(e2e_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": {k: {"launches": v["launches"], "ms_per_launch": round(v["ms_per_launch"], 4),
+                        "gbs": round(v["gbs"], 1)} for k, v in per_kernel.items()},
+        "event_ms_per_step": round(event_total_ms / args.steps, 4),
+        "error_bits": err,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(X, eps, ms)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -13,12 +13,13 @@
 
 #include <torch/extension.h>
 
+#include <c10/core/DeviceGuard.h>
 #include <c10/hip/HIPFunctions.h>
-#include <c10/hip/HIPGuard.h>
 #include <c10/hip/HIPStream.h>
 
 #include <map>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -57,7 +58,7 @@ DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, fl
                 "(gfx950); pass a tensor on a ROCm device or run on a machine with one.");
     device = c10::Device(c10::kCUDA, c10::hip::current_device());
   }
-  c10::hip::HIPGuard guard(device);
+  c10::DeviceGuard guard(device);
   DeviceRun r;
   r.Xd = X.to(device).contiguous();
   const int64_t n = r.Xd.size(0);
@@ -157,6 +158,63 @@ std::vector<torch::Tensor> dbscan2d_hip_debug(torch::Tensor X, double eps, int64
   return out;
 }
 
+// Benchmark helper (no reference counterpart): `steps` back-to-back calls on
+// one resident input, with HIP events on the launch stream before every
+// kernel.  Returns (labels, per-kernel mean ms over the steps, kernel names,
+// event-measured ms from the first kernel of step 0 to the end of the last).
+std::tuple<torch::Tensor, std::vector<double>, std::vector<std::string>, double>
+dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t steps) {
+  check_input(X, eps, min_samples);
+  TORCH_CHECK(X.is_cuda() && X.size(0) > 0 && steps >= 1,
+              "dbscan2d_benchmark: needs a non-empty ROCm tensor and steps >= 1");
+  c10::DeviceGuard guard(X.device());
+  auto Xd = X.contiguous();
+  const int64_t n = Xd.size(0);
+  size_t ws_bytes = 0;
+  check_status(dbscan2d_hip_workspace_bytes(n, &ws_bytes));
+  auto opts = torch::TensorOptions().device(X.device());
+  auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
+  auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
+  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  constexpr int K = DBSCAN_HIP_NUM_KERNELS + 1;
+  std::vector<hipEvent_t> ev(static_cast<size_t>(steps) * K, nullptr);
+  auto destroy = [&]() {
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
+  };
+  for (auto& x : ev) {
+    if (hipEventCreate(&x) != hipSuccess) {
+      destroy();
+      TORCH_CHECK(false, "dbscan2d_benchmark: hipEventCreate failed");
+    }
+  }
+  for (int64_t s = 0; s < steps; ++s) {
+    const int rc = dbscan2d_hip_timed(Xd.data_ptr<float>(), n, eps, min_samples,
+                                      labels.data_ptr<int32_t>(), workspace.data_ptr(), ws_bytes,
+                                      stream, ev.data() + s * K);
+    if (rc != DBSCAN_HIP_OK) destroy();
+    check_status(rc);
+  }
+  if (hipEventSynchronize(ev.back()) != hipSuccess) {
+    destroy();
+    TORCH_CHECK(false, "dbscan2d_benchmark: hipEventSynchronize failed");
+  }
+  std::vector<double> mean(DBSCAN_HIP_NUM_KERNELS, 0.0);
+  for (int64_t s = 0; s < steps; ++s) {
+    for (int k = 0; k < DBSCAN_HIP_NUM_KERNELS; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[s * K + k], ev[s * K + k + 1]);
+      mean[k] += ms / static_cast<double>(steps);
+    }
+  }
+  float total = 0.f;
+  (void)hipEventElapsedTime(&total, ev.front(), ev.back());
+  destroy();
+  std::vector<std::string> names;
+  for (int k = 0; k < DBSCAN_HIP_NUM_KERNELS; ++k) names.emplace_back(dbscan2d_hip_kernel_name(k));
+  return {labels, mean, names, static_cast<double>(total)};
+}
+
 // csrc/extension.cpp:65-81
 PYBIND11_MODULE(_C, m) {
   m.doc() = "Native PyTorch-ROCm DBSCAN (2D, grid-hashed) on gfx950.";
@@ -171,4 +229,8 @@ PYBIND11_MODULE(_C, m) {
         pybind11::arg("min_samples"),
         "Debug variant: returns [labels, info, perm, xs, ys, point_cell, is_core, cell_start, "
         "cell_key, cell_flags, cell_ref, uf, cell_cluster]. Not part of the public API.");
+  m.def("dbscan2d_benchmark", &dbscan2d_benchmark, pybind11::arg("X"), pybind11::arg("eps"),
+        pybind11::arg("min_samples"), pybind11::arg("steps"),
+        "Benchmark helper: (labels, per-kernel mean ms, kernel names, total event ms) over "
+        "`steps` back-to-back calls. Not part of the public API.");
 }

@@ -907,8 +907,38 @@ T* at(void* ws, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
 }
 
+// Launch order of the kernels; dbscan2d_hip_timed records an event before
+// each of them and one after the last.
+enum KernelSlot : int {
+  kKBounds = 0,
+  kKHist,
+  kKSort0,
+  kKCells = kKSort0 + kMaxPasses,
+  kKNgh,
+  kKCore,
+  kKUnion,
+  kKFlatten,
+  kKRank,
+  kKCluster,
+  kKLabel,
+  kNumKernels
+};
+static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header");
+
+const char* const kKernelNames[kNumKernels] = {
+    "k_bounds", "k_hist", "k_sort_pass[0]", "k_sort_pass[1]", "k_sort_pass[2]",
+    "k_sort_pass[3]", "k_sort_pass[4]", "k_sort_pass[5]", "k_sort_pass[6]", "k_sort_pass[7]",
+    "k_cells", "k_ngh", "k_core", "k_union", "k_flatten", "k_rank", "k_cluster", "k_label"};
+
+// First kernel of each reference stage (stages 7-9 and 9-10 share kernels:
+// core labels, border assignment and unpermute are one fused kernel).
+constexpr int kStageFirstKernel[DBSCAN_HIP_NUM_STAGES + 1] = {
+    kKBounds, kKSort0, kKCells, kKNgh, kKCore, kKUnion, kKRank, kKLabel, kKLabel,
+    kNumKernels, kNumKernels};
+
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
-        void* workspace, size_t workspace_bytes, hipStream_t stream, float* stage_ms) {
+        void* workspace, size_t workspace_bytes, hipStream_t stream, float* stage_ms,
+        hipEvent_t* kev) {
   if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
   if (!(eps > 0.0)) return DBSCAN_HIP_ERR_EPS;  // csrc/extension.cpp:29
   if (min_samples64 < 1) return DBSCAN_HIP_ERR_MIN_SAMPLES;  // csrc/extension.cpp:30
@@ -931,10 +961,13 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const float best_init = eps_sq + 1.0f;
 
   hipEvent_t ev[DBSCAN_HIP_NUM_STAGES + 1] = {};
-  auto fail = [&](hipError_t e) {
-    g_last_hip_error = static_cast<int>(e);
+  auto release = [&]() {
     for (auto& x : ev)
       if (x) (void)hipEventDestroy(x);
+  };
+  auto fail = [&](hipError_t e) {
+    g_last_hip_error = static_cast<int>(e);
+    release();
     return DBSCAN_HIP_ERR_HIP;
   };
   if (stage_ms) {
@@ -943,14 +976,27 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
       if (e != hipSuccess) return fail(e);
     }
   }
-  auto mark = [&](int s) -> hipError_t {
-    return stage_ms ? hipEventRecord(ev[s], stream) : hipSuccess;
+  // Records the events due before kernel slot k (k == kNumKernels: the end).
+  auto before = [&](int k) -> hipError_t {
+    hipError_t e = hipSuccess;
+    if (kev) e = hipEventRecord(kev[k], stream);
+    if (stage_ms)
+      for (int s = 0; s <= DBSCAN_HIP_NUM_STAGES && e == hipSuccess; ++s)
+        if (kStageFirstKernel[s] == k) e = hipEventRecord(ev[s], stream);
+    return e;
   };
+  hipError_t e;
 
   if (ms < 1) {  // dbscan_grid.hpp:141: every point is noise
+    for (int k = 0; k < kNumKernels; ++k)
+      if ((e = before(k)) != hipSuccess) return fail(e);
     k_fill_i32<<<grid_for(n, 256, 4096), 256, 0, stream>>>(labels, n, kNoise);
-    const hipError_t e = hipGetLastError();
-    return e == hipSuccess ? DBSCAN_HIP_OK : fail(e);
+    if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if (stage_ms && (e = hipEventSynchronize(ev[DBSCAN_HIP_NUM_STAGES])) != hipSuccess)
+      return fail(e);
+    release();
+    return DBSCAN_HIP_OK;
   }
 
   void* ws = workspace;
@@ -980,21 +1026,22 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
   const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
-  hipError_t e;
 
   // Stage 1: cell keys (bounds + digit histograms).
-  if ((e = mark(0)) != hipSuccess) return fail(e);
+  if ((e = before(kKBounds)) != hipSuccess) return fail(e);
   const int64_t zero16 = static_cast<int64_t>((L.zero_end - kParamsZeroBegin) / 16);
   k_bounds<<<g_reduce, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials),
                                          at<uint4>(ws, kParamsZeroBegin), zero16);
+  if ((e = before(kKHist)) != hipSuccess) return fail(e);
   k_hist<<<g_hist, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm);
 
   // Stage 2: radix passes (sort + gather in one).
-  if ((e = mark(1)) != hipSuccess) return fail(e);
+  if ((e = before(kKSort0)) != hipSuccess) return fail(e);
   k_sort_pass<true><<<g_sort, kSortThreads, 0, stream>>>(X2, nullptr, nullptr, nullptr, xs_a,
                                                          ys_a, idx_a, prm, sort_status, 0, n,
                                                          inv_r);
   for (int p = 1; p < kMaxPasses; ++p) {
+    if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
     k_sort_pass<false><<<g_sort, kSortThreads, 0, stream>>>(
         nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
@@ -1003,39 +1050,39 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   }
 
   // Stage 3: cell runs.
-  if ((e = mark(2)) != hipSuccess) return fail(e);
+  if ((e = before(kKCells)) != hipSuccess) return fail(e);
   k_cells<<<g_scan, kScanThreads, 0, stream>>>(xs_a, ys_a, xs_b, ys_b, prm,
                                                at<uint64_t>(ws, L.cell_status), point_cell,
                                                cell_start, cell_key, cell_flags, n, inv_r);
 
   // Stage 4: window columns.
-  if ((e = mark(3)) != hipSuccess) return fail(e);
+  if ((e = before(kKNgh)) != hipSuccess) return fail(e);
   k_ngh<<<g_cells, 256, 0, stream>>>(prm, cell_key, ngh, cell_ref, uf);
 
   // Stage 5: core marking.
-  if ((e = mark(4)) != hipSuccess) return fail(e);
+  if ((e = before(kKCore)) != hipSuccess) return fail(e);
   k_core<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start, ngh,
                                       is_core, cell_flags, n, ms, eps_sq);
 
   // Stage 6: union-find.
-  if ((e = mark(5)) != hipSuccess) return fail(e);
+  if ((e = before(kKUnion)) != hipSuccess) return fail(e);
   k_union<<<g_cells, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
                                        cell_flags, is_core, uf, eps_sq);
+  if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
 
   // Stage 7: cluster ids.
-  if ((e = mark(6)) != hipSuccess) return fail(e);
+  if ((e = before(kKRank)) != hipSuccess) return fail(e);
   k_rank<<<g_scan, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank);
+  if ((e = before(kKCluster)) != hipSuccess) return fail(e);
   k_cluster<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank, cell_cluster);
 
   // Stages 8-10: core labels, border assignment and unpermute, fused.
-  if ((e = mark(7)) != hipSuccess) return fail(e);
-  if ((e = mark(8)) != hipSuccess) return fail(e);
+  if ((e = before(kKLabel)) != hipSuccess) return fail(e);
   k_label<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, point_cell,
                                        cell_start, ngh, cell_flags, is_core, cell_cluster, labels,
                                        n, eps_sq, best_init);
-  if ((e = mark(9)) != hipSuccess) return fail(e);
-  if ((e = mark(10)) != hipSuccess) return fail(e);
+  if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
   if (stage_ms) {
@@ -1043,8 +1090,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
     for (int s = 0; s < DBSCAN_HIP_NUM_STAGES; ++s) {
       if ((e = hipEventElapsedTime(&stage_ms[s], ev[s], ev[s + 1])) != hipSuccess) return fail(e);
     }
-    for (auto& x : ev) (void)hipEventDestroy(x);
   }
+  release();
   return DBSCAN_HIP_OK;
 }
 
@@ -1061,14 +1108,28 @@ int dbscan2d_hip_workspace_bytes(int64_t n, size_t* bytes) {
 
 int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples, int32_t* labels,
                  void* workspace, size_t workspace_bytes, hipStream_t stream) {
-  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr);
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
+             nullptr);
+}
+
+int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_samples,
+                       int32_t* labels, void* workspace, size_t workspace_bytes,
+                       hipStream_t stream, hipEvent_t* events) {
+  if (!events) return DBSCAN_HIP_ERR_NULL;
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
+             events);
+}
+
+const char* dbscan2d_hip_kernel_name(int i) {
+  return (i >= 0 && i < kNumKernels) ? kKernelNames[i] : nullptr;
 }
 
 int dbscan2d_hip_profile(const float* X, int64_t n, double eps, int64_t min_samples,
                          int32_t* labels, void* workspace, size_t workspace_bytes,
                          hipStream_t stream, float* stage_ms) {
   if (!stage_ms) return DBSCAN_HIP_ERR_NULL;
-  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, stage_ms);
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, stage_ms,
+             nullptr);
 }
 
 const char* dbscan2d_hip_stage_name(int i) {

@@ -78,6 +78,22 @@ int dbscan2d_hip_profile(const float* X, int64_t n, double eps, int64_t min_samp
 /* Name of stage i (0 <= i < DBSCAN_HIP_NUM_STAGES), or NULL. */
 const char* dbscan2d_hip_stage_name(int i);
 
+/* Number of kernel launch slots of one call (sort passes beyond the key width
+ * are launched and return at once), and their names. */
+#define DBSCAN_HIP_NUM_KERNELS 18
+const char* dbscan2d_hip_kernel_name(int i);
+
+/*
+ * Same as dbscan2d_hip, and records events[k] on `stream` before kernel slot
+ * k and events[DBSCAN_HIP_NUM_KERNELS] after the last one (the caller creates
+ * the DBSCAN_HIP_NUM_KERNELS + 1 events).  Does not synchronise, so calls can
+ * be timed back to back.  No counterpart in the reference: per-kernel timing
+ * for the benchmark's roofline figures.
+ */
+int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_samples,
+                       int32_t* labels, void* workspace, size_t workspace_bytes,
+                       hipStream_t stream, hipEvent_t* events);
+
 /*
  * Byte offsets, inside the workspace of a dbscan2d_hip call on n points, of
  * the intermediates that call leaves behind (all in the library's sorted
@@ -88,7 +104,9 @@ const char* dbscan2d_hip_stage_name(int i);
  * the `info` block.
  */
 typedef struct {
-  size_t info;        /* int32[8]: kxmin, kxmax, kymin, kymax, num_cells, npasses, 0, 0 */
+  size_t info;        /* int32[8]: kxmin, kxmax, kymin, kymax, num_cells, npasses,
+                         error bits (0 when every bounded wait and index check
+                         passed), 0 */
   size_t xs_a, ys_a, idx_a;   /* sorted x, y, original index if npasses is odd  */
   size_t xs_b, ys_b, idx_b;   /* ... if npasses is even                         */
   size_t point_cell;  /* int32[n]  cell of each sorted point                      */
