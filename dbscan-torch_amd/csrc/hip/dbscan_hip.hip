@@ -488,7 +488,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     const float* __restrict__ xs_b, const float* __restrict__ ys_b, Params* __restrict__ prm,
     uint64_t* __restrict__ status, int32_t* __restrict__ point_cell,
     int32_t* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
-    uint32_t* __restrict__ cell_flags, int64_t n, float inv_r) {
+    uint32_t* __restrict__ cell_flags, uint4* __restrict__ cell_box, int64_t n, float inv_r) {
   __shared__ uint64_t s_last[kScanThreads];
   __shared__ uint32_t s_scan[kScanThreads / 64];
   __shared__ uint32_t s_tile;
@@ -556,6 +556,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
         cell_start[c] = static_cast<int32_t>(e);
         cell_key[c] = key[j];
         cell_flags[c] = 0u;
+        cell_box[c] = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);  // empty box for k_bbox
       }
       point_cell[e] = static_cast<int32_t>(c);
       if (e == n - 1) {
@@ -629,95 +630,297 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
   }
 }
 
+// Ordered-integer image of a float (monotone for non-NaN values), so that
+// per-cell bounds can use integer atomicMin / atomicMax.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Cell bounding box [xmin, xmax, ymin, ymax] decoded from its ordered form.
+__device__ __forceinline__ float4 load_box(const uint4* __restrict__ cell_box, int32_t c) {
+  const uint4 u = cell_box[c];
+  return make_float4(ord2f(u.x), ord2f(u.y), ord2f(u.z), ord2f(u.w));
+}
+
+// Squared distance from a point to a box, as the reference prunes with it
+// (dbscan_grid.hpp:119-127): by monotone rounding it never exceeds the
+// computed d2 of any point inside the box, so "> eps_sq" proves that no point
+// of the cell is within eps.  fmaxf drops NaN terms, which only weakens the
+// prune.
+__device__ __forceinline__ float point_box_d2(float x, float y, float4 b) {
+  const float dx = fmaxf(fmaxf(b.x - x, x - b.y), 0.0f);
+  const float dy = fmaxf(fmaxf(b.z - y, y - b.w), 0.0f);
+  return dx * dx + dy * dy;
+}
+
+// dbscan_grid.hpp:129-134
+__device__ __forceinline__ float box_box_d2(float4 a, float4 b) {
+  const float dx = fmaxf(fmaxf(b.x - a.y, a.x - b.y), 0.0f);
+  const float dy = fmaxf(fmaxf(b.z - a.w, a.z - b.w), 0.0f);
+  return dx * dx + dy * dy;
+}
+
+__device__ __forceinline__ int32_t bcast(int32_t v, int src) {
+  return __builtin_amdgcn_readlane(v, src);
+}
+__device__ __forceinline__ float bcastf(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ int lowest_lane(uint64_t m) {
+  return __ffsll(static_cast<unsigned long long>(m)) - 1;
+}
+
+// Per-cell bounding boxes of the non-NaN points (the reference's per-cell
+// boxes, dbscan_grid.hpp:332-349, used only for sound prunes).  One thread per
+// sorted point; each wave reduces its runs of equal cells with shuffles.  A
+// cell that lies wholly inside the wave is stored directly; a run that touches
+// either end of the wave is merged with atomics into the sentinel written by
+// k_cells.
+constexpr int32_t kBigCell = 64;  // cells larger than this get their own box test
+
+__global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
+                                              const float* __restrict__ xs_a,
+                                              const float* __restrict__ ys_a,
+                                              const float* __restrict__ xs_b,
+                                              const float* __restrict__ ys_b,
+                                              const int32_t* __restrict__ point_cell,
+                                              uint4* __restrict__ cell_box, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool in_a = (prm->npasses & 1) != 0;
+  const float* __restrict__ xs = in_a ? xs_a : xs_b;
+  const float* __restrict__ ys = in_a ? ys_a : ys_b;
+  const bool valid = i < n;
+  int32_t c = -1;
+  uint32_t mnx = 0xFFFFFFFFu, mxx = 0u, mny = 0xFFFFFFFFu, mxy = 0u;
+  if (valid) {
+    c = point_cell[i];
+    const float x = xs[i], y = ys[i];
+    if (!isnan(x) && !isnan(y)) {
+      mnx = mxx = f2ord(x);
+      mny = mxy = f2ord(y);
+    }
+  }
+  // Suffix reduction within runs of equal c (runs are contiguous: sorted).
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t oc = __shfl_down(c, d, 64);
+    const uint32_t a0 = __shfl_down(mnx, d, 64), a1 = __shfl_down(mxx, d, 64);
+    const uint32_t a2 = __shfl_down(mny, d, 64), a3 = __shfl_down(mxy, d, 64);
+    if (lane + d < 64 && oc == c) {
+      mnx = min(mnx, a0); mxx = max(mxx, a1);
+      mny = min(mny, a2); mxy = max(mxy, a3);
+    }
+  }
+  const int32_t prev_c = __shfl_up(c, 1, 64);
+  const bool head = valid && (lane == 0 || prev_c != c);
+  // Lane of the last point of each run: the run ends inside the wave if the
+  // next lane holds another cell (or is past n).
+  const int32_t next_c = __shfl_down(c, 1, 64);
+  const bool tail = valid && (lane == 63 || next_c != c);
+  const uint64_t tails = __ballot(tail);
+  if (head) {
+    // the run's last lane: first tail at or after this lane
+    const uint64_t after = tails & ~((1ull << lane) - 1ull);
+    const int last = lowest_lane(after);
+    const bool starts_here = lane > 0 || i == 0;
+    const bool ends_here = last < 63 || i + (last - lane) == n - 1;
+    if (starts_here && ends_here) {
+      cell_box[c] = make_uint4(mnx, mxx, mny, mxy);
+    } else {
+      if (mnx != 0xFFFFFFFFu) {
+        atomicMin(&cell_box[c].x, mnx);
+        atomicMax(&cell_box[c].y, mxx);
+        atomicMin(&cell_box[c].z, mny);
+        atomicMax(&cell_box[c].w, mxy);
+      }
+    }
+  }
+}
+
+// Counts, for the point (xi, yi), the points of [p0, p1) with
+// fl(fl(ex*ex) + fl(ey*ey)) <= eps_sq, 64 per step, until `total` reaches ms.
+__device__ __forceinline__ void count_range(const float* __restrict__ xs,
+                                            const float* __restrict__ ys, int32_t p0, int32_t p1,
+                                            float xi, float yi, float eps_sq, int32_t ms,
+                                            int32_t& total) {
+  const int lane = threadIdx.x & 63;
+  for (int32_t base = p0; base < p1 && total < ms; base += 64) {
+    const int32_t j = base + lane;
+    bool hit = false;
+    if (j < p1) {
+      const float ex = xs[j] - xi, ey = ys[j] - yi;
+      hit = ex * ex + ey * ey <= eps_sq;
+    }
+    total += static_cast<int32_t>(__popcll(__ballot(hit)));
+  }
+}
+
 // Core marking (dbscan_grid.hpp:267-321).  A point in a cell of at least
-// min_samples points is core; otherwise its own cell counts wholesale and the
-// other window cells count the points with fl(fl(ex*ex) + fl(ey*ey)) <= eps_sq.
+// min_samples points is core (pass 1).  Otherwise (pass 2) its own cell counts
+// wholesale and the other window cells count their points with
+// fl(fl(ex*ex) + fl(ey*ey)) <= eps_sq; the whole wave counts for one such
+// point at a time, 64 candidates per step, stopping once min_samples is
+// reached.  Window cells are walked in runs; cells above kBigCell points are
+// skipped when their box is farther than eps.
 __global__ __launch_bounds__(256) void k_core(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const float* __restrict__ xs_b,
     const float* __restrict__ ys_b, const int32_t* __restrict__ point_cell,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
-    uint8_t* __restrict__ is_core, uint32_t* __restrict__ cell_flags, int64_t n,
-    int32_t min_samples, float eps_sq) {
+    const uint4* __restrict__ cell_box, uint8_t* __restrict__ is_core,
+    uint32_t* __restrict__ cell_flags, int64_t n, int32_t min_samples, float eps_sq) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int lane = threadIdx.x & 63;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  const int32_t c = point_cell[i];
-  const int32_t s = cell_start[c], e = cell_start[c + 1];
-  int32_t cnt = e - s;
-  bool core;
-  if (cnt >= min_samples) {
-    core = true;
-    if (i == s) cell_flags[c] = 1u;
-  } else {
-    const float xi = xs[i], yi = ys[i];
-    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-    for (int col = 0; col < 5 && cnt < min_samples; ++col) {
+  const bool valid = i < n;
+  int32_t c = 0, s = 0, e = 0;
+  if (valid) {
+    c = point_cell[i];
+    s = cell_start[c];
+    e = cell_start[c + 1];
+  }
+  const bool full = valid && (e - s) >= min_samples;
+  const bool need = valid && !full;
+  float x = 0.f, y = 0.f;
+  if (need) {
+    x = xs[i];
+    y = ys[i];
+  }
+  bool core = full;
+  uint64_t todo = __ballot(need);
+  while (todo) {
+    const int src = lowest_lane(todo);
+    todo &= todo - 1ull;
+    const int32_t ci = bcast(c, src), si = bcast(s, src), ei = bcast(e, src);
+    const float xi = bcastf(x, src), yi = bcastf(y, src);
+    int32_t total = ei - si;  // own cell, wholesale (dbscan_grid.hpp:295-297)
+    const int32_t* row = ngh + static_cast<int64_t>(ci) * 10;
+    for (int col = 0; col < 5 && total < min_samples; ++col) {
       const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
       if (j0 >= j1) continue;
-      const int32_t p0 = cell_start[j0], p1 = cell_start[j1];
-      for (int32_t j = p0; j < p1; ++j) {
-        if (j == s) {  // own cell: counted wholesale above
-          j = e - 1;
-          continue;
-        }
-        const float ex = xs[j] - xi, ey = ys[j] - yi;
-        if (ex * ex + ey * ey <= eps_sq) {
-          if (++cnt >= min_samples) break;
-        }
+      int32_t run = cell_start[j0];
+      for (int32_t nc = j0; nc < j1 && total < min_samples; ++nc) {
+        const int32_t cs = cell_start[nc], ce = cell_start[nc + 1];
+        const bool own = nc == ci;
+        const bool big = ce - cs > kBigCell;
+        if (!own && !big) continue;
+        count_range(xs, ys, run, cs, xi, yi, eps_sq, min_samples, total);
+        run = ce;
+        if (own) continue;
+        if (point_box_d2(xi, yi, load_box(cell_box, nc)) > eps_sq) continue;
+        count_range(xs, ys, cs, ce, xi, yi, eps_sq, min_samples, total);
       }
+      count_range(xs, ys, run, cell_start[j1], xi, yi, eps_sq, min_samples, total);
     }
-    core = cnt >= min_samples;
+    if (lane == src) core = total >= min_samples;
+  }
+  if (!valid) return;
+  is_core[i] = core ? 1 : 0;
+  if (full) {
+    if (i == s) cell_flags[c] = 1u;
+  } else {
     atomicOr(cell_flags + c, core ? 1u : 2u);
   }
-  is_core[i] = core ? 1 : 0;
 }
 
-// Cell-level union over eps-close core pairs (dbscan_grid.hpp:351-387).  Each
-// unordered window pair is visited once, from its smaller internal index:
-// the rest of the own column above c, and the two columns to the right.
+// True if some core point of [sa, ea) and some core point of [sb, eb) are
+// within eps (dbscan_grid.hpp:370-384).  Points are first filtered by their
+// distance to the other cell's box; the surviving pairs are tested 64 at a
+// time, and the search stops at the first hit.
+__device__ bool core_pair_within(const float* __restrict__ xs, const float* __restrict__ ys,
+                                 const uint8_t* __restrict__ is_core, int32_t sa, int32_t ea,
+                                 float4 box_a, int32_t sb, int32_t eb, float4 box_b,
+                                 float eps_sq) {
+  const int lane = threadIdx.x & 63;
+  for (int32_t ba = sa; ba < ea; ba += 64) {
+    const int32_t a = ba + lane;
+    float xa = 0.f, ya = 0.f;
+    bool va = false;
+    if (a < ea && is_core[a]) {
+      xa = xs[a];
+      ya = ys[a];
+      va = point_box_d2(xa, ya, box_b) <= eps_sq;
+    }
+    const uint64_t ma = __ballot(va);
+    if (!ma) continue;
+    for (int32_t bb = sb; bb < eb; bb += 64) {
+      const int32_t b = bb + lane;
+      float xb = 0.f, yb = 0.f;
+      bool vb = false;
+      if (b < eb && is_core[b]) {
+        xb = xs[b];
+        yb = ys[b];
+        vb = point_box_d2(xb, yb, box_a) <= eps_sq;
+      }
+      if (!__ballot(vb)) continue;
+      uint64_t m = ma;
+      while (m) {
+        const int src = lowest_lane(m);
+        m &= m - 1ull;
+        const float xi = bcastf(xa, src), yi = bcastf(ya, src);
+        const float ex = xb - xi, ey = yb - yi;
+        if (__ballot(vb && ex * ex + ey * ey <= eps_sq)) return true;
+      }
+    }
+  }
+  return false;
+}
+
+// Cell-level union over eps-close core pairs (dbscan_grid.hpp:351-387).  One
+// wave per cell c; each unordered window pair is visited once, from its
+// smaller internal index (the rest of c's column above c, and the two columns
+// to the right).  Lanes screen the up to 12 neighbours in parallel (has a
+// core, boxes within eps); the wave then runs the pair search for each
+// survivor not already in c's component.
 __global__ __launch_bounds__(256) void k_union(
     Params* __restrict__ prm, const float* __restrict__ xs_a, const float* __restrict__ ys_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
     const int32_t* __restrict__ cell_ref, const uint32_t* __restrict__ cell_flags,
-    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, float eps_sq) {
+    const uint4* __restrict__ cell_box, const uint8_t* __restrict__ is_core,
+    int32_t* __restrict__ uf, float eps_sq) {
   const int32_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  for (int64_t c64 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c64 < C;
-       c64 += static_cast<int64_t>(gridDim.x) * 256) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+  const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
+  for (int64_t c64 = wave; c64 < C; c64 += nwaves) {
     const int32_t c = static_cast<int32_t>(c64);
     if (!(cell_flags[c] & 1u)) continue;
+    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+    // forward neighbours: (c, row[5]) in column dx=0, [row[6], row[7]),
+    // [row[8], row[9]) in dx=1, 2
+    const int32_t n0 = row[5] - (c + 1), n1 = row[7] - row[6], n2 = row[9] - row[8];
+    int32_t nc = -1;
+    if (lane < n0) nc = c + 1 + lane;
+    else if (lane < n0 + n1) nc = row[6] + (lane - n0);
+    else if (lane < n0 + n1 + n2) nc = row[8] + (lane - n0 - n1);
+    const float4 box_c = load_box(cell_box, c);
+    bool cand = false;
+    if (nc >= 0 && (cell_flags[nc] & 1u))
+      cand = box_box_d2(box_c, load_box(cell_box, nc)) <= eps_sq;
+    uint64_t m = __ballot(cand);
+    if (!m) continue;
     const int32_t rc = cell_ref[c];
     const int32_t s_c = cell_start[c], e_c = cell_start[c + 1];
-    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-    for (int col = 2; col < 5; ++col) {
-      const int32_t j0 = (col == 2) ? c + 1 : row[2 * col];
-      const int32_t j1 = row[2 * col + 1];
-      for (int32_t nc = j0; nc < j1; ++nc) {
-        if (!(cell_flags[nc] & 1u)) continue;
-        const int32_t rn = cell_ref[nc];
-        if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
-        const int32_t s_n = cell_start[nc], e_n = cell_start[nc + 1];
-        bool found = false;
-        for (int32_t i = s_c; i < e_c && !found; ++i) {
-          if (!is_core[i]) continue;
-          const float xi = xs[i], yi = ys[i];
-          for (int32_t j = s_n; j < e_n; ++j) {
-            if (!is_core[j]) continue;
-            const float ex = xs[j] - xi, ey = ys[j] - yi;
-            if (ex * ex + ey * ey <= eps_sq) {
-              found = true;
-              break;
-            }
-          }
-        }
-        if (found) uf_unite(uf, rc, rn, &prm->error);
-      }
+    while (m) {
+      const int src = lowest_lane(m);
+      m &= m - 1ull;
+      const int32_t nb = bcast(nc, src);
+      const int32_t rn = cell_ref[nb];
+      if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
+      const bool hit = core_pair_within(xs, ys, is_core, s_c, e_c, box_c, cell_start[nb],
+                                        cell_start[nb + 1], load_box(cell_box, nb), eps_sq);
+      if (hit && lane == 0) uf_unite(uf, rc, rn, &prm->error);
     }
   }
 }
@@ -788,51 +991,106 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
   }
 }
 
+// Best core candidate among [p0, p1) for the point (xi, yi): lanes compute
+// d2 (cores only); a candidate is accepted when d2 <= eps_sq and d2 < best_d2,
+// and the smallest accepted d2 of the step (lowest lane on ties: the earliest
+// in scan order) becomes the new best.  Steps run in scan order, so the result
+// is the first strict minimum, as in the reference's sequential loop.
+__device__ __forceinline__ void best_in_range(const float* __restrict__ xs,
+                                              const float* __restrict__ ys,
+                                              const uint8_t* __restrict__ is_core,
+                                              const int32_t* __restrict__ point_cell,
+                                              const int32_t* __restrict__ cell_cluster,
+                                              int32_t p0, int32_t p1, float xi, float yi,
+                                              float eps_sq, float& best_d2, int32_t& best_lab) {
+  const int lane = threadIdx.x & 63;
+  for (int32_t base = p0; base < p1; base += 64) {
+    const int32_t j = base + lane;
+    bool acc = false;
+    float d2 = 0.f;
+    if (j < p1 && is_core[j]) {
+      const float ex = xs[j] - xi, ey = ys[j] - yi;
+      d2 = ex * ex + ey * ey;
+      acc = d2 <= eps_sq && d2 < best_d2;
+    }
+    if (!__ballot(acc)) continue;
+    uint64_t key = acc ? ((static_cast<uint64_t>(__float_as_uint(d2)) << 32) | lane) : ~0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const uint64_t o = __shfl_xor(key, d, 64);
+      key = o < key ? o : key;
+    }
+    best_d2 = __uint_as_float(static_cast<uint32_t>(key >> 32));
+    best_lab = cell_cluster[point_cell[base + static_cast<int32_t>(key & 63u)]];
+  }
+}
+
 // Core labels, border assignment and unpermute (dbscan_grid.hpp:401-455).
 // A non-core point takes the label of the first core j, in window-offset
 // order, with d2 <= eps_sq and d2 strictly below the best so far (initially
-// eps_sq + 1.0f), i.e. the lexicographic minimum of (d2, off).
+// eps_sq + 1.0f), i.e. the lexicographic minimum of (d2, off).  The whole wave
+// searches for one non-core point at a time, over the window's columns in
+// order; cells above kBigCell points are skipped when they hold no core or
+// their box is farther than eps.
 __global__ __launch_bounds__(256) void k_label(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const uint32_t* __restrict__ idx_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b,
     const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
-    const uint32_t* __restrict__ cell_flags, const uint8_t* __restrict__ is_core,
-    const int32_t* __restrict__ cell_cluster, int32_t* __restrict__ labels, int64_t n,
-    float eps_sq, float best_init) {
+    const uint32_t* __restrict__ cell_flags, const uint4* __restrict__ cell_box,
+    const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
+    int32_t* __restrict__ labels, int64_t n, float eps_sq, float best_init) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
+  const int lane = threadIdx.x & 63;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
-  const int32_t c = point_cell[i];
-  int32_t lab;
-  if (is_core[i]) {
-    lab = cell_cluster[c];
-  } else {
-    lab = kNoise;
-    float best = best_init;
-    const float xi = xs[i], yi = ys[i];
-    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-    for (int col = 0; col < 5; ++col) {
-      const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
-      for (int32_t nc = j0; nc < j1; ++nc) {
-        if (!(cell_flags[nc] & 1u)) continue;
-        const int32_t p1 = cell_start[nc + 1];
-        for (int32_t j = cell_start[nc]; j < p1; ++j) {
-          if (!is_core[j]) continue;
-          const float ex = xs[j] - xi, ey = ys[j] - yi;
-          const float d2 = ex * ex + ey * ey;
-          if (d2 <= eps_sq && d2 < best) {
-            best = d2;
-            lab = cell_cluster[nc];
-          }
-        }
-      }
+  const bool valid = i < n;
+  int32_t c = 0, lab = kNoise;
+  bool need = false;
+  float x = 0.f, y = 0.f;
+  if (valid) {
+    c = point_cell[i];
+    if (is_core[i]) {
+      lab = cell_cluster[c];
+    } else {
+      need = true;
+      x = xs[i];
+      y = ys[i];
     }
   }
+  uint64_t todo = __ballot(need);
+  while (todo) {
+    const int src = lowest_lane(todo);
+    todo &= todo - 1ull;
+    const int32_t ci = bcast(c, src);
+    const float xi = bcastf(x, src), yi = bcastf(y, src);
+    float best_d2 = best_init;
+    int32_t best_lab = kNoise;
+    const int32_t* row = ngh + static_cast<int64_t>(ci) * 10;
+    for (int col = 0; col < 5; ++col) {
+      const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
+      if (j0 >= j1) continue;
+      int32_t run = cell_start[j0];
+      for (int32_t nc = j0; nc < j1; ++nc) {
+        const int32_t cs = cell_start[nc], ce = cell_start[nc + 1];
+        if (ce - cs <= kBigCell) continue;
+        best_in_range(xs, ys, is_core, point_cell, cell_cluster, run, cs, xi, yi, eps_sq,
+                      best_d2, best_lab);
+        run = ce;
+        if (!(cell_flags[nc] & 1u)) continue;
+        if (point_box_d2(xi, yi, load_box(cell_box, nc)) > eps_sq) continue;
+        best_in_range(xs, ys, is_core, point_cell, cell_cluster, cs, ce, xi, yi, eps_sq,
+                      best_d2, best_lab);
+      }
+      best_in_range(xs, ys, is_core, point_cell, cell_cluster, run, cell_start[j1], xi, yi,
+                    eps_sq, best_d2, best_lab);
+    }
+    if (lane == src) lab = best_lab;
+  }
+  if (!valid) return;
   const uint32_t dst = idx[i];
   if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
 }
@@ -845,7 +1103,7 @@ struct Layout {
   size_t params, sort_status, cell_status, rank_status, zero_end;
   size_t partials;
   size_t xs_a, ys_a, idx_a, xs_b, ys_b, idx_b;
-  size_t point_cell, is_core, cell_start, cell_key, cell_flags, ngh, cell_ref, uf, rank,
+  size_t point_cell, is_core, cell_start, cell_key, cell_flags, cell_box, ngh, cell_ref, uf, rank,
       cell_cluster;
   size_t total;
 };
@@ -880,6 +1138,7 @@ Layout make_layout(int64_t n) {
   L.cell_start = take((N + 1) * 4);
   L.cell_key = take(N * 8);
   L.cell_flags = take(N * 4);
+  L.cell_box = take(N * 16);
   L.ngh = take(N * 10 * 4);
   L.cell_ref = take(N * 4);
   L.uf = take(N * 4);
@@ -914,6 +1173,7 @@ enum KernelSlot : int {
   kKHist,
   kKSort0,
   kKCells = kKSort0 + kMaxPasses,
+  kKBbox,
   kKNgh,
   kKCore,
   kKUnion,
@@ -928,7 +1188,7 @@ static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header
 const char* const kKernelNames[kNumKernels] = {
     "k_bounds", "k_hist", "k_sort_pass[0]", "k_sort_pass[1]", "k_sort_pass[2]",
     "k_sort_pass[3]", "k_sort_pass[4]", "k_sort_pass[5]", "k_sort_pass[6]", "k_sort_pass[7]",
-    "k_cells", "k_ngh", "k_core", "k_union", "k_flatten", "k_rank", "k_cluster", "k_label"};
+    "k_cells", "k_bbox", "k_ngh", "k_core", "k_union", "k_flatten", "k_rank", "k_cluster", "k_label"};
 
 // First kernel of each reference stage (stages 7-9 and 9-10 share kernels:
 // core labels, border assignment and unpermute are one fused kernel).
@@ -1014,6 +1274,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   int32_t* cell_start = at<int32_t>(ws, L.cell_start);
   uint64_t* cell_key = at<uint64_t>(ws, L.cell_key);
   uint32_t* cell_flags = at<uint32_t>(ws, L.cell_flags);
+  uint4* cell_box = at<uint4>(ws, L.cell_box);
   int32_t* ngh = at<int32_t>(ws, L.ngh);
   int32_t* cell_ref = at<int32_t>(ws, L.cell_ref);
   int32_t* uf = at<int32_t>(ws, L.uf);
@@ -1026,6 +1287,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
   const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
+  const int g_union = grid_for(n, 4 * 4, 16384);  // one wave per cell, up to 64k waves
 
   // Stage 1: cell keys (bounds + digit histograms).
   if ((e = before(kKBounds)) != hipSuccess) return fail(e);
@@ -1053,7 +1315,10 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
   k_cells<<<g_scan, kScanThreads, 0, stream>>>(xs_a, ys_a, xs_b, ys_b, prm,
                                                at<uint64_t>(ws, L.cell_status), point_cell,
-                                               cell_start, cell_key, cell_flags, n, inv_r);
+                                               cell_start, cell_key, cell_flags, cell_box, n,
+                                               inv_r);
+  if ((e = before(kKBbox)) != hipSuccess) return fail(e);
+  k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_box, n);
 
   // Stage 4: window columns.
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
@@ -1062,12 +1327,12 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   // Stage 5: core marking.
   if ((e = before(kKCore)) != hipSuccess) return fail(e);
   k_core<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start, ngh,
-                                      is_core, cell_flags, n, ms, eps_sq);
+                                      cell_box, is_core, cell_flags, n, ms, eps_sq);
 
   // Stage 6: union-find.
   if ((e = before(kKUnion)) != hipSuccess) return fail(e);
-  k_union<<<g_cells, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                       cell_flags, is_core, uf, eps_sq);
+  k_union<<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
+                                       cell_flags, cell_box, is_core, uf, eps_sq);
   if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
 
@@ -1080,8 +1345,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   // Stages 8-10: core labels, border assignment and unpermute, fused.
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
   k_label<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, point_cell,
-                                       cell_start, ngh, cell_flags, is_core, cell_cluster, labels,
-                                       n, eps_sq, best_init);
+                                       cell_start, ngh, cell_flags, cell_box, is_core, cell_cluster,
+                                       labels, n, eps_sq, best_init);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
