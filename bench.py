@@ -38,18 +38,23 @@ HBM_PEAK_GBS = 8000.0
 #   k_sort_pass[0]  reads X (8) and writes x, y, index (12)
 #   k_sort_pass[p]  reads and writes x, y, index (12 + 12)
 #   k_label         reads point_cell (4), is_core (1), index (4), writes the
-#                   label (4); border points' window reads are not counted
-#   k_core          reads point_cell (4), writes is_core (1); cell_start 8/cell
+#                   label (4)
+#   k_border        per cell flags and start (12); window reads not counted
+#   k_core_sparse   reads cell_start and flags per cell (12); window reads of
+#                   sparse cells' points are not counted
 #   k_cells         reads x, y (8), writes point_cell (4); start+key 12/cell
-#   k_bbox          reads x, y, point_cell (12), writes a 16-byte box per cell
+#   k_bbox          reads x, y, point_cell (12), writes is_core (1) and a
+#                   16-byte box per cell
 KERNEL_BYTES = {
     "k_sort_pass[0]": (20, 0),
     "k_sort_pass[1]": (24, 0), "k_sort_pass[2]": (24, 0), "k_sort_pass[3]": (24, 0),
     "k_sort_pass[4]": (24, 0), "k_sort_pass[5]": (24, 0), "k_sort_pass[6]": (24, 0),
     "k_sort_pass[7]": (24, 0),
     "k_bounds": (8, 0), "k_hist": (8, 0),
-    "k_cells": (12, 12), "k_bbox": (12, 16), "k_core": (5, 8), "k_label": (13, 4),
-    "k_ngh": (0, 48), "k_union": (9, 12), "k_flatten": (0, 12), "k_rank": (0, 8),
+    "k_cells": (12, 12), "k_bbox": (13, 16), "k_core_sparse": (0, 12),
+    "k_label": (13, 0), "k_border": (0, 12),
+    "k_ngh": (0, 48), "k_union[3x3]": (9, 12), "k_flatten[3x3]": (0, 12),
+    "k_union[rest]": (0, 24), "k_flatten": (0, 12), "k_rank": (0, 8),
     "k_cluster": (0, 16),
 }
 

@@ -51,14 +51,19 @@ constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 8;           // 64-bit keys at most
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 12;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kMaxReduceBlocks = 512;
-constexpr int kCellTileCounter = kMaxPasses;      // tile_counter slot of k_cells
-constexpr int kRankTileCounter = kMaxPasses + 1;  // tile_counter slot of k_rank
+// kTileOrder: the look-back kernels (k_sort_pass, k_cells, k_rank) use the
+// workgroup id as the tile index.  Workgroups of a 1-D grid are dispatched in
+// increasing id order, so a tile only ever waits on tiles that are already
+// resident or finished.  (A contended atomic ticket costs ~10 ns per
+// workgroup here, which is tens of microseconds for these grids.)  Every wait
+// is bounded by kSpinLimit and reports kErrSortSpin / kErrScanSpin instead of
+// hanging.
 constexpr uint32_t kSpinLimit = 1u << 26;         // bounded waits; see ErrFlag
 constexpr uint64_t kCountMask = (1ull << 48) - 1;
 
@@ -344,54 +349,107 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
   }
 }
 
+// Sum of the earlier tiles' counts of one digit (decoupled look-back).  The
+// status words of four predecessors are read per round trip; a word tagged
+// `tag_pre` is an inclusive prefix (stop), `tag_agg` a tile's own count
+// (add, keep going), anything else is not yet published (or left by an
+// earlier pass) and is waited for, within kSpinLimit.
+__device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ status,
+                                                   int64_t tile, int digit, uint64_t tag_agg,
+                                                   uint64_t tag_pre, int32_t* error) {
+  uint64_t excl = 0;
+  int64_t j = tile - 1;
+  uint32_t spins = 0;
+  while (j >= 0) {
+    uint64_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      w[k] = (j - k >= 0) ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadix + digit)
+                          : tag_pre;
+    // consume the published words in order: `used` of them, `stop` at a prefix
+    int used = 0;
+    bool stop = false, blocked = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint64_t tag = w[k] & ~kCountMask;
+      const bool ready = tag == tag_agg || tag == tag_pre;
+      if (!stop && !blocked) {
+        if (ready) {
+          excl += w[k] & kCountMask;
+          ++used;
+          stop = tag == tag_pre;
+        } else {
+          blocked = true;  // not yet published
+        }
+      }
+    }
+    if (stop) break;
+    j -= used;
+    if (used == 0) {
+      if (++spins > kSpinLimit) {
+        atomicOr(error, static_cast<int32_t>(kErrSortSpin));
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return excl;
+}
+
 // One stable LSD radix pass over the digit (key >> 8*pass) & 255.
 // kFirst: read the interleaved input X and emit original indices.
+// A tile of kSortTile points is ranked in LDS (wave64 ballot matching into
+// per-wave counters, item by item, which keeps equal digits in input order),
+// offset by the global digit histogram and a look-back over earlier tiles,
+// then reordered by digit through one LDS buffer, array by array, so that runs
+// of equal digits leave as contiguous writes.
 template <bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
-    int pass, int64_t n, float inv_r) {
+    int pass, int64_t n64, float inv_r) {
+  const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31: 32-bit offsets
   const KeyMap m = load_keymap(prm);
   if (pass >= m.npasses) return;
 
-  __shared__ float s_x[kSortTile];
-  __shared__ float s_y[kSortTile];
-  __shared__ uint32_t s_i[kSortTile];
+  __shared__ uint32_t s_buf[kSortTile];
   __shared__ uint8_t s_d[kSortTile];
   __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
   __shared__ uint32_t s_goff[kRadix];
   __shared__ uint32_t s_dstart[kRadix];
   __shared__ uint32_t s_scan[kSortThreads / 64];
-  __shared__ uint32_t s_tile;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
 #pragma unroll
   for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][t] = 0u;
-  if (t == 0) s_tile = atomicAdd(&prm->tile_counter[pass], 1u);
   uint32_t htot;
   const uint32_t goff = block_excl_scan<kSortThreads>(prm->hist[pass][t], htot, s_scan);
-  const uint32_t tile = s_tile;
-  const int64_t base = static_cast<int64_t>(tile) * kSortTile;
-  const int64_t wbase = base + static_cast<int64_t>(wave) * (64 * kSortItems);
+  const uint32_t tile = blockIdx.x;  // see kTileOrder
+  const uint32_t base = tile * kSortTile;
+  const uint32_t wbase = base + static_cast<uint32_t>(wave) * (64 * kSortItems);
   const int shift = kRadixBits * pass;
 
   float x[kSortItems], y[kSortItems];
   uint32_t id[kSortItems], dg[kSortItems], rk[kSortItems];
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    const int64_t e = wbase + k * 64 + lane;
-    x[k] = 0.f; y[k] = 0.f; id[k] = 0u; dg[k] = 0u;
-    if (e < n) {
-      if (kFirst) {
-        const float2 p = X[e];
-        x[k] = p.x; y[k] = p.y; id[k] = static_cast<uint32_t>(e);
-      } else {
-        x[k] = in_x[e]; y[k] = in_y[e]; id[k] = in_i[e];
-      }
-      dg[k] = static_cast<uint32_t>(sort_key(x[k], y[k], inv_r, m) >> shift) & (kRadix - 1);
+    // unconditional loads (a clamped index past the end) keep the per-item
+    // values in plain registers; validity is tested where it matters
+    const uint32_t e = wbase + k * 64 + lane;
+    const uint32_t ec = e < n ? e : n - 1;
+    if (kFirst) {
+      const float2 p = X[ec];
+      x[k] = p.x; y[k] = p.y; id[k] = e;
+    } else {
+      x[k] = in_x[ec]; y[k] = in_y[ec]; id[k] = in_i[ec];
     }
   }
+  // all loads of the tile are in flight before the first key is computed
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k)
+    dg[k] = static_cast<uint32_t>(sort_key(x[k], y[k], inv_r, m) >> shift) & (kRadix - 1);
 
   // Rank within the wave's segment, item by item (stable): lanes holding the
   // same digit are found with 8 ballots; the lowest such lane bumps the wave's
@@ -399,7 +457,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    const bool valid = (wbase + k * 64 + lane) < n;
+    const bool valid = wbase + k * 64 + lane < n;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kRadixBits; ++b) {
@@ -429,54 +487,40 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   uint32_t ttot;
   const uint32_t dstart = block_excl_scan<kSortThreads>(dtot, ttot, s_scan);
   s_dstart[t] = dstart;
-
-  // Look-back over earlier tiles for digit t.  Status words of earlier passes
-  // carry smaller tags and read as "not yet published".
   uint64_t excl = 0;
   if (tile > 0) {
-    int64_t j = static_cast<int64_t>(tile) - 1;
-    uint32_t spins = 0;
-    while (j >= 0) {
-      const uint64_t s = ld_acquire(status + static_cast<uint64_t>(j) * kRadix + t);
-      const uint64_t tag = s & ~kCountMask;
-      if (tag != tag_agg && tag != tag_pre) {
-        if (++spins > kSpinLimit) {
-          atomicOr(&prm->error, static_cast<int32_t>(kErrSortSpin));
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      excl += s & kCountMask;
-      if (tag == tag_pre) break;
-      --j;
-    }
+    excl = digit_lookback(status, static_cast<int64_t>(tile), t, tag_agg, tag_pre, &prm->error);
     st_release(my_status, tag_pre | (excl + dtot));
   }
   s_goff[t] = goff + static_cast<uint32_t>(excl) - dstart;
   __syncthreads();
 
-  // Reorder the tile in LDS by digit, then write runs of equal digits to
-  // consecutive global positions.
+  // Tile-local position of every item (kept in rk), then the digits in
+  // position order.
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    if (wbase + k * 64 + lane < n) {
-      const uint32_t d = dg[k];
-      const uint32_t li = s_dstart[d] + s_wcnt[wave][d] + rk[k];
-      s_x[li] = x[k]; s_y[li] = y[k]; s_i[li] = id[k]; s_d[li] = static_cast<uint8_t>(d);
-    }
+    rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
+    if (wbase + k * 64 + lane < n) s_d[rk[k]] = static_cast<uint8_t>(dg[k]);
   }
-  __syncthreads();
-  const int tile_n = static_cast<int>(min(static_cast<int64_t>(kSortTile), n - base));
-  for (int j = t; j < tile_n; j += kSortThreads) {
-    const uint32_t pos = s_goff[s_d[j]] + static_cast<uint32_t>(j);
-    if (pos >= static_cast<uint64_t>(n)) {
-      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
-      continue;
+  const int tile_n = static_cast<int>(min(static_cast<uint32_t>(kSortTile), n - base));
+  // x, y and index in turn through the one buffer
+#pragma unroll
+  for (int arr = 0; arr < 3; ++arr) {
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+      if (wbase + k * 64 + lane < n)
+        s_buf[rk[k]] = arr == 0 ? __float_as_uint(x[k]) : (arr == 1 ? __float_as_uint(y[k]) : id[k]);
     }
-    out_x[pos] = s_x[j];
-    out_y[pos] = s_y[j];
-    out_i[pos] = s_i[j];
+    __syncthreads();
+    uint32_t* __restrict__ out = arr == 0 ? reinterpret_cast<uint32_t*>(out_x)
+                                          : (arr == 1 ? reinterpret_cast<uint32_t*>(out_y) : out_i);
+#pragma unroll 2
+    for (int j = t; j < tile_n; j += kSortThreads) {
+      const uint32_t pos = s_goff[s_d[j]] + static_cast<uint32_t>(j);
+      if (pos < n) out[pos] = s_buf[j];
+      else if (arr == 0) atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+    }
+    __syncthreads();
   }
 }
 
@@ -491,16 +535,13 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     uint32_t* __restrict__ cell_flags, uint4* __restrict__ cell_box, int64_t n, float inv_r) {
   __shared__ uint64_t s_last[kScanThreads];
   __shared__ uint32_t s_scan[kScanThreads / 64];
-  __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
   const KeyMap m = load_keymap(prm);
   const bool in_a = (m.npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const int t = threadIdx.x;
-  if (t == 0) s_tile = atomicAdd(&prm->tile_counter[kCellTileCounter], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = blockIdx.x;  // see kTileOrder
   const int64_t base = static_cast<int64_t>(tile) * kScanTile;
   const int64_t e0 = base + static_cast<int64_t>(t) * kScanItems;
 
@@ -675,12 +716,15 @@ __device__ __forceinline__ int lowest_lane(uint64_t m) {
 }
 
 // Per-cell bounding boxes of the non-NaN points (the reference's per-cell
-// boxes, dbscan_grid.hpp:332-349, used only for sound prunes).  One thread per
-// sorted point; each wave reduces its runs of equal cells with shuffles.  A
-// cell that lies wholly inside the wave is stored directly; a run that touches
-// either end of the wave is merged with atomics into the sentinel written by
-// k_cells.
-constexpr int32_t kBigCell = 64;  // cells larger than this get their own box test
+// boxes, dbscan_grid.hpp:332-349, used only for sound prunes), and core
+// marking pass 1 (dbscan_grid.hpp:270-279): every point of a cell with at
+// least min_samples points is core.  One thread per sorted point; each wave
+// reduces its runs of equal cells with shuffles.  A cell that lies wholly
+// inside the wave is stored directly; a run that touches either end of the
+// wave is merged with atomics into the sentinel written by k_cells.
+// cell_flags bits: kCellHasNaN and, for full cells, kCellHasCore are set by
+// k_bbox; k_core_sparse sets the core bits of the other cells.
+constexpr uint32_t kCellHasCore = 1u, kCellHasNonCore = 2u, kCellHasNaN = 4u;
 
 __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               const float* __restrict__ xs_a,
@@ -688,7 +732,11 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               const float* __restrict__ xs_b,
                                               const float* __restrict__ ys_b,
                                               const int32_t* __restrict__ point_cell,
-                                              uint4* __restrict__ cell_box, int64_t n) {
+                                              const int32_t* __restrict__ cell_start,
+                                              uint4* __restrict__ cell_box,
+                                              uint32_t* __restrict__ cell_flags,
+                                              uint8_t* __restrict__ is_core, int64_t n,
+                                              int32_t min_samples) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool in_a = (prm->npasses & 1) != 0;
@@ -697,12 +745,18 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
   const bool valid = i < n;
   int32_t c = -1;
   uint32_t mnx = 0xFFFFFFFFu, mxx = 0u, mny = 0xFFFFFFFFu, mxy = 0u;
+  int32_t has_nan = 0;
+  bool full = false;
   if (valid) {
     c = point_cell[i];
+    full = cell_start[c + 1] - cell_start[c] >= min_samples;
+    is_core[i] = full ? 1 : 0;  // points of other cells are decided by k_core_sparse
     const float x = xs[i], y = ys[i];
     if (!isnan(x) && !isnan(y)) {
       mnx = mxx = f2ord(x);
       mny = mxy = f2ord(y);
+    } else {
+      has_nan = 1;
     }
   }
   // Suffix reduction within runs of equal c (runs are contiguous: sorted).
@@ -711,9 +765,11 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
     const int32_t oc = __shfl_down(c, d, 64);
     const uint32_t a0 = __shfl_down(mnx, d, 64), a1 = __shfl_down(mxx, d, 64);
     const uint32_t a2 = __shfl_down(mny, d, 64), a3 = __shfl_down(mxy, d, 64);
+    const int32_t a4 = __shfl_down(has_nan, d, 64);
     if (lane + d < 64 && oc == c) {
       mnx = min(mnx, a0); mxx = max(mxx, a1);
       mny = min(mny, a2); mxy = max(mxy, a3);
+      has_nan |= a4;
     }
   }
   const int32_t prev_c = __shfl_up(c, 1, 64);
@@ -723,6 +779,8 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
   const int32_t next_c = __shfl_down(c, 1, 64);
   const bool tail = valid && (lane == 63 || next_c != c);
   const uint64_t tails = __ballot(tail);
+  if (head && (has_nan || full))
+    atomicOr(cell_flags + c, (has_nan ? kCellHasNaN : 0u) | (full ? kCellHasCore : 0u));
   if (head) {
     // the run's last lane: first tail at or after this lane
     const uint64_t after = tails & ~((1ull << lane) - 1ull);
@@ -740,6 +798,59 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
       }
     }
   }
+}
+
+// Cells up to this many points are scanned by a single lane; larger cells are
+// handed to the whole wave.
+constexpr int32_t kLaneScan = 48;
+// A cell pair whose point-count product is at most this is tested by one lane.
+constexpr int32_t kLanePairs = 1024;
+
+// Largest computed squared distance from (x, y) to any point of the box:
+// |fl(q - x)| is at most |fl(b - x)| for the box edge b on q's side (monotone
+// rounding), so every point of the box has d2 <= this value.  Valid only for
+// cells without NaN points (their box ignores them).
+__device__ __forceinline__ float far_box_d2(float x, float y, float4 b) {
+  const float dx = fmaxf(fabsf(b.x - x), fabsf(b.y - x));
+  const float dy = fmaxf(fabsf(b.z - y), fabsf(b.w - y));
+  return dx * dx + dy * dy;
+}
+
+// Every point of cell nc is within eps of (x, y): box test, finite, no NaN point.
+__device__ __forceinline__ bool cell_inside(float x, float y, float4 b, uint32_t flags,
+                                            float eps_sq) {
+  const float far = far_box_d2(x, y, b);
+  return !(flags & kCellHasNaN) && far <= eps_sq && far < __builtin_inff();
+}
+
+// The cells of one window column: at most 5 consecutive cells [j0, j0 + cnt)
+// (one per y' in [y'-2, y'+2]).  Their starts, flags and boxes are loaded
+// together so that a lane waits once per column rather than once per cell.
+struct WinColumn {
+  int32_t j0, cnt;
+  int32_t start[6];
+  uint32_t flags[5];
+  float4 box[5];
+};
+
+__device__ __forceinline__ void load_column(const int32_t* __restrict__ cell_start,
+                                            const uint32_t* __restrict__ cell_flags,
+                                            const uint4* __restrict__ cell_box, int32_t j0,
+                                            int32_t j1, WinColumn& w) {
+  w.j0 = j0;
+  w.cnt = j1 > j0 ? j1 - j0 : 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w.start[k] = (k <= w.cnt && w.cnt > 0) ? cell_start[j0 + k] : 0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    w.flags[k] = k < w.cnt ? cell_flags[j0 + k] : 0u;
+    w.box[k] = k < w.cnt ? load_box(cell_box, j0 + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Columns of the window nearest first (counting order is free).
+__device__ __forceinline__ int near_col(int k) {
+  return k == 0 ? 2 : (k == 1 ? 1 : (k == 2 ? 3 : (k == 3 ? 0 : 4)));
 }
 
 // Counts, for the point (xi, yi), the points of [p0, p1) with
@@ -760,80 +871,156 @@ __device__ __forceinline__ void count_range(const float* __restrict__ xs,
   }
 }
 
-// Core marking (dbscan_grid.hpp:267-321).  A point in a cell of at least
-// min_samples points is core (pass 1).  Otherwise (pass 2) its own cell counts
-// wholesale and the other window cells count their points with
-// fl(fl(ex*ex) + fl(ey*ey)) <= eps_sq; the whole wave counts for one such
-// point at a time, 64 candidates per step, stopping once min_samples is
-// reached.  Window cells are walked in runs; cells above kBigCell points are
-// skipped when their box is farther than eps.
-__global__ __launch_bounds__(256) void k_core(
+// Pass-2 count for one point by one lane (dbscan_grid.hpp:284-307): the own
+// cell counts wholesale (`total` on entry); window cells whose box is farther
+// than eps add nothing, cells whose box lies within eps add all their points,
+// small cells are scanned.  Sets `deferred` when a big cell would need a scan.
+__device__ __forceinline__ int32_t lane_core_count(
+    const float* __restrict__ xs, const float* __restrict__ ys, float x, float y, int32_t c,
+    const int32_t* __restrict__ row, const int32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ cell_flags, const uint4* __restrict__ cell_box,
+    int32_t min_samples, float eps_sq, int32_t total, bool& deferred) {
+  for (int k = 0; k < 5 && total < min_samples; ++k) {
+    const int col = near_col(k);
+    WinColumn w;
+    load_column(cell_start, cell_flags, cell_box, row[2 * col], row[2 * col + 1], w);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (q >= w.cnt || total >= min_samples || w.j0 + q == c) continue;
+      const int32_t cs = w.start[q], ce = w.start[q + 1];
+      if (point_box_d2(x, y, w.box[q]) > eps_sq) continue;
+      if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq)) {
+        total += ce - cs;
+      } else if (ce - cs <= kLaneScan) {
+        for (int32_t j = cs; j < ce; ++j) {
+          const float ex = xs[j] - x, ey = ys[j] - y;
+          if (ex * ex + ey * ey <= eps_sq && ++total >= min_samples) break;
+        }
+      } else {
+        deferred = true;
+      }
+    }
+  }
+  return total;
+}
+
+// The same count by the whole wave for one point (xi, yi) of cell ci: small
+// cells are scanned in runs, big cells after their box tests, 64 candidates
+// per step, until min_samples.
+__device__ int32_t wave_core_count(const float* __restrict__ xs, const float* __restrict__ ys,
+                                   float xi, float yi, int32_t ci,
+                                   const int32_t* __restrict__ row,
+                                   const int32_t* __restrict__ cell_start,
+                                   const uint32_t* __restrict__ cell_flags,
+                                   const uint4* __restrict__ cell_box, int32_t min_samples,
+                                   float eps_sq, int32_t total) {
+  for (int k = 0; k < 5 && total < min_samples; ++k) {
+    const int col = near_col(k);
+    const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
+    if (j0 >= j1) continue;
+    int32_t run = cell_start[j0];
+    for (int32_t nc = j0; nc < j1 && total < min_samples; ++nc) {
+      const int32_t cs = cell_start[nc], ce = cell_start[nc + 1];
+      const bool own = nc == ci;
+      if (!own && ce - cs <= kLaneScan) continue;
+      count_range(xs, ys, run, cs, xi, yi, eps_sq, min_samples, total);
+      run = ce;
+      if (own) continue;
+      const float4 b = load_box(cell_box, nc);
+      if (point_box_d2(xi, yi, b) > eps_sq) continue;
+      if (cell_inside(xi, yi, b, cell_flags[nc], eps_sq)) {
+        total += ce - cs;
+        continue;
+      }
+      count_range(xs, ys, cs, ce, xi, yi, eps_sq, min_samples, total);
+    }
+    count_range(xs, ys, run, cell_start[j1], xi, yi, eps_sq, min_samples, total);
+  }
+  return total;
+}
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), src);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), src);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Core marking pass 2 (dbscan_grid.hpp:281-307) for the cells with fewer than
+// min_samples points (pass 1 was done by k_bbox).  One lane per cell counts
+// for each of the cell's points in turn; the counts stop at min_samples,
+// which cannot change the outcome.  Points whose count needs a scan of a big
+// cell, and cells of more than 64 points (possible when min_samples > 64),
+// are counted by the whole wave.  Sets the cell's core bits.
+__global__ __launch_bounds__(256) void k_core_sparse(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const float* __restrict__ xs_b,
-    const float* __restrict__ ys_b, const int32_t* __restrict__ point_cell,
-    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
-    const uint4* __restrict__ cell_box, uint8_t* __restrict__ is_core,
-    uint32_t* __restrict__ cell_flags, int64_t n, int32_t min_samples, float eps_sq) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+    const float* __restrict__ ys_b, const int32_t* __restrict__ cell_start,
+    const int32_t* __restrict__ ngh, const uint4* __restrict__ cell_box,
+    uint8_t* __restrict__ is_core, uint32_t* __restrict__ cell_flags, int32_t min_samples,
+    float eps_sq) {
+  const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  const bool valid = i < n;
-  int32_t c = 0, s = 0, e = 0;
-  if (valid) {
-    c = point_cell[i];
-    s = cell_start[c];
-    e = cell_start[c + 1];
-  }
-  const bool full = valid && (e - s) >= min_samples;
-  const bool need = valid && !full;
-  float x = 0.f, y = 0.f;
-  if (need) {
-    x = xs[i];
-    y = ys[i];
-  }
-  bool core = full;
-  uint64_t todo = __ballot(need);
-  while (todo) {
-    const int src = lowest_lane(todo);
-    todo &= todo - 1ull;
-    const int32_t ci = bcast(c, src), si = bcast(s, src), ei = bcast(e, src);
-    const float xi = bcastf(x, src), yi = bcastf(y, src);
-    int32_t total = ei - si;  // own cell, wholesale (dbscan_grid.hpp:295-297)
-    const int32_t* row = ngh + static_cast<int64_t>(ci) * 10;
-    for (int col = 0; col < 5 && total < min_samples; ++col) {
-      const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
-      if (j0 >= j1) continue;
-      int32_t run = cell_start[j0];
-      for (int32_t nc = j0; nc < j1 && total < min_samples; ++nc) {
-        const int32_t cs = cell_start[nc], ce = cell_start[nc + 1];
-        const bool own = nc == ci;
-        const bool big = ce - cs > kBigCell;
-        if (!own && !big) continue;
-        count_range(xs, ys, run, cs, xi, yi, eps_sq, min_samples, total);
-        run = ce;
-        if (own) continue;
-        if (point_box_d2(xi, yi, load_box(cell_box, nc)) > eps_sq) continue;
-        count_range(xs, ys, cs, ce, xi, yi, eps_sq, min_samples, total);
-      }
-      count_range(xs, ys, run, cell_start[j1], xi, yi, eps_sq, min_samples, total);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+  const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
+  for (int64_t base = wave * 64; base < C; base += nwaves * 64) {
+    const int32_t c = static_cast<int32_t>(base + lane);
+    const bool valid = base + lane < C;
+    int32_t s = 0, e = 0;
+    if (valid) {
+      s = cell_start[c];
+      e = cell_start[c + 1];
     }
-    if (lane == src) core = total >= min_samples;
-  }
-  if (!valid) return;
-  is_core[i] = core ? 1 : 0;
-  if (full) {
-    if (i == s) cell_flags[c] = 1u;
-  } else {
-    atomicOr(cell_flags + c, core ? 1u : 2u);
+    const bool sparse = valid && e - s < min_samples;
+    const bool whole_wave = sparse && e - s > 64;
+    uint64_t pending = 0;
+    uint32_t bits = 0;
+    if (sparse && !whole_wave) {
+      const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+      for (int32_t p = s; p < e; ++p) {
+        bool deferred = false;
+        const int32_t tot = lane_core_count(xs, ys, xs[p], ys[p], c, row, cell_start, cell_flags,
+                                            cell_box, min_samples, eps_sq, e - s, deferred);
+        if (tot >= min_samples) {
+          is_core[p] = 1;
+          bits |= kCellHasCore;
+        } else if (deferred) {
+          pending |= 1ull << (p - s);
+        } else {
+          is_core[p] = 0;
+          bits |= kCellHasNonCore;
+        }
+      }
+    }
+    uint64_t todo = __ballot(pending != 0 || whole_wave);
+    while (todo) {
+      const int src = lowest_lane(todo);
+      todo &= todo - 1ull;
+      const int32_t cc = bcast(c, src), cs = bcast(s, src), ce = bcast(e, src);
+      const uint64_t pm = bcast64(pending, src);
+      const bool all = ce - cs > 64;
+      const int32_t* row = ngh + static_cast<int64_t>(cc) * 10;
+      for (int32_t p = cs; p < ce; ++p) {
+        if (!all && !((pm >> (p - cs)) & 1ull)) continue;
+        const int32_t tot = wave_core_count(xs, ys, xs[p], ys[p], cc, row, cell_start,
+                                            cell_flags, cell_box, min_samples, eps_sq, ce - cs);
+        if (lane == src) {
+          const bool core = tot >= min_samples;
+          is_core[p] = core ? 1 : 0;
+          bits |= core ? kCellHasCore : kCellHasNonCore;
+        }
+      }
+    }
+    if (sparse) cell_flags[c] |= bits;  // the lane owns this cell's core bits
   }
 }
 
 // True if some core point of [sa, ea) and some core point of [sb, eb) are
-// within eps (dbscan_grid.hpp:370-384).  Points are first filtered by their
-// distance to the other cell's box; the surviving pairs are tested 64 at a
-// time, and the search stops at the first hit.
+// within eps (dbscan_grid.hpp:370-384), for the whole wave: points are first
+// filtered by their distance to the other cell's box, the surviving pairs are
+// tested 64 at a time, and the search stops at the first hit.
 __device__ bool core_pair_within(const float* __restrict__ xs, const float* __restrict__ ys,
                                  const uint8_t* __restrict__ is_core, int32_t sa, int32_t ea,
                                  float4 box_a, int32_t sb, int32_t eb, float4 box_b,
@@ -873,53 +1060,112 @@ __device__ bool core_pair_within(const float* __restrict__ xs, const float* __re
   return false;
 }
 
+// The same test for one lane (small cell pairs).
+__device__ __forceinline__ bool core_pair_within_lane(const float* __restrict__ xs,
+                                                      const float* __restrict__ ys,
+                                                      const uint8_t* __restrict__ is_core,
+                                                      int32_t sa, int32_t ea, int32_t sb,
+                                                      int32_t eb, float4 box_b, float eps_sq) {
+  for (int32_t a = sa; a < ea; ++a) {
+    if (!is_core[a]) continue;
+    const float xa = xs[a], ya = ys[a];
+    if (point_box_d2(xa, ya, box_b) > eps_sq) continue;
+    for (int32_t b = sb; b < eb; ++b) {
+      if (!is_core[b]) continue;
+      const float ex = xs[b] - xa, ey = ys[b] - ya;
+      if (ex * ex + ey * ey <= eps_sq) return true;
+    }
+  }
+  return false;
+}
+
 // Cell-level union over eps-close core pairs (dbscan_grid.hpp:351-387).  One
-// wave per cell c; each unordered window pair is visited once, from its
-// smaller internal index (the rest of c's column above c, and the two columns
-// to the right).  Lanes screen the up to 12 neighbours in parallel (has a
-// core, boxes within eps); the wave then runs the pair search for each
-// survivor not already in c's component.
+// lane per (cell c, forward neighbour slot): each unordered window pair is
+// visited once, from its smaller internal index (the rest of c's column above
+// c, and the two columns to the right; at most 12 slots).  Pairs with a core
+// on both sides and boxes within eps are tested by the lane when small, and
+// by the whole wave otherwise.
+// Two launches split the pairs: kPhase 0 takes the 3x3 neighbours (dx <= 1,
+// |dy| <= 1), which join almost every dense region; after a flatten, kPhase 1
+// takes the rest and first skips pairs whose cells already share a root.
+// The union is order-independent, so the split cannot change a label.
+constexpr int kUnionSlots = 12;
+
+template <int kPhase>
 __global__ __launch_bounds__(256) void k_union(
     Params* __restrict__ prm, const float* __restrict__ xs_a, const float* __restrict__ ys_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
     const int32_t* __restrict__ cell_ref, const uint32_t* __restrict__ cell_flags,
-    const uint4* __restrict__ cell_box, const uint8_t* __restrict__ is_core,
-    int32_t* __restrict__ uf, float eps_sq) {
-  const int32_t C = prm->num_cells;
+    const uint4* __restrict__ cell_box, const uint64_t* __restrict__ cell_key,
+    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, float eps_sq) {
+  const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const int lane = threadIdx.x & 63;
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
-  for (int64_t c64 = wave; c64 < C; c64 += nwaves) {
-    const int32_t c = static_cast<int32_t>(c64);
-    if (!(cell_flags[c] & 1u)) continue;
-    const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-    // forward neighbours: (c, row[5]) in column dx=0, [row[6], row[7]),
-    // [row[8], row[9]) in dx=1, 2
-    const int32_t n0 = row[5] - (c + 1), n1 = row[7] - row[6], n2 = row[9] - row[8];
-    int32_t nc = -1;
-    if (lane < n0) nc = c + 1 + lane;
-    else if (lane < n0 + n1) nc = row[6] + (lane - n0);
-    else if (lane < n0 + n1 + n2) nc = row[8] + (lane - n0 - n1);
-    const float4 box_c = load_box(cell_box, c);
-    bool cand = false;
-    if (nc >= 0 && (cell_flags[nc] & 1u))
-      cand = box_box_d2(box_c, load_box(cell_box, nc)) <= eps_sq;
-    uint64_t m = __ballot(cand);
-    if (!m) continue;
-    const int32_t rc = cell_ref[c];
-    const int32_t s_c = cell_start[c], e_c = cell_start[c + 1];
-    while (m) {
-      const int src = lowest_lane(m);
-      m &= m - 1ull;
-      const int32_t nb = bcast(nc, src);
-      const int32_t rn = cell_ref[nb];
+  const int64_t total = C * kUnionSlots;
+  for (int64_t base = wave * 64; base < total; base += nwaves * 64) {
+    const int64_t t = base + lane;
+    int32_t c = -1, nc = -1;
+    if (t < total) {
+      c = static_cast<int32_t>(t / kUnionSlots);
+      const int slot = static_cast<int>(t - static_cast<int64_t>(c) * kUnionSlots);
+      if (cell_flags[c] & kCellHasCore) {
+        const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+        const int32_t n0 = row[5] - (c + 1), n1 = row[7] - row[6], n2 = row[9] - row[8];
+        int dx = 0;
+        if (slot < n0) {
+          nc = c + 1 + slot;
+        } else if (slot < n0 + n1) {
+          nc = row[6] + (slot - n0);
+          dx = 1;
+        } else if (slot < n0 + n1 + n2) {
+          nc = row[8] + (slot - n0 - n1);
+          dx = 2;
+        }
+        if (nc >= 0) {
+          const int64_t dy = static_cast<int64_t>(static_cast<uint32_t>(cell_key[nc])) -
+                             static_cast<int64_t>(static_cast<uint32_t>(cell_key[c]));
+          const bool near3 = dx <= 1 && dy >= -1 && dy <= 1;
+          if (near3 != (kPhase == 0)) nc = -1;
+        }
+      }
+    }
+    bool cand = false, deferred = false;
+    float4 box_c = make_float4(0.f, 0.f, 0.f, 0.f), box_n = box_c;
+    int32_t sa = 0, ea = 0, sb = 0, eb = 0;
+    if (nc >= 0 && (cell_flags[nc] & kCellHasCore)) {
+      // after the first phase's flatten, uf holds roots: equal roots need no test
+      if (kPhase == 1 && uf[cell_ref[c]] == uf[cell_ref[nc]]) nc = -1;
+    }
+    if (nc >= 0 && (cell_flags[nc] & kCellHasCore)) {
+      box_c = load_box(cell_box, c);
+      box_n = load_box(cell_box, nc);
+      cand = box_box_d2(box_c, box_n) <= eps_sq;
+    }
+    if (cand) {
+      sa = cell_start[c]; ea = cell_start[c + 1];
+      sb = cell_start[nc]; eb = cell_start[nc + 1];
+      if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs) {
+        if (core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq))
+          uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
+      } else {
+        deferred = true;
+      }
+    }
+    uint64_t todo = __ballot(deferred);
+    while (todo) {
+      const int src = lowest_lane(todo);
+      todo &= todo - 1ull;
+      const int32_t cw = bcast(c, src), nw = bcast(nc, src);
+      const int32_t rc = cell_ref[cw], rn = cell_ref[nw];
       if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
-      const bool hit = core_pair_within(xs, ys, is_core, s_c, e_c, box_c, cell_start[nb],
-                                        cell_start[nb + 1], load_box(cell_box, nb), eps_sq);
+      const float4 bc = load_box(cell_box, cw), bn = load_box(cell_box, nw);
+      const bool hit = core_pair_within(xs, ys, is_core, bcast(sa, src), bcast(ea, src), bc,
+                                        bcast(sb, src), bcast(eb, src), bn, eps_sq);
       if (hit && lane == 0) uf_unite(uf, rc, rn, &prm->error);
     }
   }
@@ -949,15 +1195,14 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
                                                        uint64_t* __restrict__ status,
                                                        uint32_t* __restrict__ flag_rank) {
   __shared__ uint32_t s_scan[kScanThreads / 64];
-  __shared__ uint32_t s_tile;
   __shared__ uint64_t s_prefix;
   const int64_t C = prm->num_cells;
   const int t = threadIdx.x;
-  if (t == 0) s_tile = atomicAdd(&prm->tile_counter[kRankTileCounter], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  // blocks past the cell count leave before taking a tile (the grid is sized
+  // for n points; tiles are only needed for C cells)
+  if (static_cast<int64_t>(blockIdx.x) * kScanTile >= C) return;
+  const uint32_t tile = blockIdx.x;  // see kTileOrder
   const int64_t base = static_cast<int64_t>(tile) * kScanTile;
-  if (base >= C) return;  // no later tile waits on this one
   const int64_t e0 = base + static_cast<int64_t>(t) * kScanItems;
   uint32_t f[kScanItems];
   uint32_t cnt = 0;
@@ -991,108 +1236,169 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
   }
 }
 
-// Best core candidate among [p0, p1) for the point (xi, yi): lanes compute
-// d2 (cores only); a candidate is accepted when d2 <= eps_sq and d2 < best_d2,
-// and the smallest accepted d2 of the step (lowest lane on ties: the earliest
-// in scan order) becomes the new best.  Steps run in scan order, so the result
-// is the first strict minimum, as in the reference's sequential loop.
-__device__ __forceinline__ void best_in_range(const float* __restrict__ xs,
-                                              const float* __restrict__ ys,
-                                              const uint8_t* __restrict__ is_core,
-                                              const int32_t* __restrict__ point_cell,
-                                              const int32_t* __restrict__ cell_cluster,
-                                              int32_t p0, int32_t p1, float xi, float yi,
-                                              float eps_sq, float& best_d2, int32_t& best_lab) {
-  const int lane = threadIdx.x & 63;
-  for (int32_t base = p0; base < p1; base += 64) {
-    const int32_t j = base + lane;
-    bool acc = false;
-    float d2 = 0.f;
-    if (j < p1 && is_core[j]) {
-      const float ex = xs[j] - xi, ey = ys[j] - yi;
-      d2 = ex * ex + ey * ey;
-      acc = d2 <= eps_sq && d2 < best_d2;
-    }
-    if (!__ballot(acc)) continue;
-    uint64_t key = acc ? ((static_cast<uint64_t>(__float_as_uint(d2)) << 32) | lane) : ~0ull;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-      const uint64_t o = __shfl_xor(key, d, 64);
-      key = o < key ? o : key;
-    }
-    best_d2 = __uint_as_float(static_cast<uint32_t>(key >> 32));
-    best_lab = cell_cluster[point_cell[base + static_cast<int32_t>(key & 63u)]];
-  }
+// Core labels and unpermute (dbscan_grid.hpp:401-412,448-455): a core point
+// takes its cell's cluster id, written to its input position.  Non-core
+// points are written by k_border.
+__global__ __launch_bounds__(256) void k_label(
+    const Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
+    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
+    const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
+    int32_t* __restrict__ labels, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n || !is_core[i]) return;
+  const uint32_t* __restrict__ idx = (prm->npasses & 1) ? idx_a : idx_b;
+  const uint32_t dst = idx[i];
+  if (dst < static_cast<uint64_t>(n)) labels[dst] = cell_cluster[point_cell[i]];
 }
 
-// Core labels, border assignment and unpermute (dbscan_grid.hpp:401-455).
-// A non-core point takes the label of the first core j, in window-offset
-// order, with d2 <= eps_sq and d2 strictly below the best so far (initially
-// eps_sq + 1.0f), i.e. the lexicographic minimum of (d2, off).  The whole wave
-// searches for one non-core point at a time, over the window's columns in
-// order; cells above kBigCell points are skipped when they hold no core or
-// their box is farther than eps.
-__global__ __launch_bounds__(256) void k_label(
+// Border search for one point by one lane (dbscan_grid.hpp:419-445): window
+// cells in scan order (dx-major, ky ascending), skipping cells without a core
+// or with a box farther than eps; the first strict minimum of d2 over cores
+// with d2 <= eps_sq wins.  Sets `deferred` when a big cell would need a scan.
+__device__ __forceinline__ int32_t lane_border(
+    const float* __restrict__ xs, const float* __restrict__ ys,
+    const uint8_t* __restrict__ is_core, float x, float y, const int32_t* __restrict__ row,
+    const int32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_flags,
+    const uint4* __restrict__ cell_box, const int32_t* __restrict__ cell_cluster, float eps_sq,
+    float best_init, bool& deferred) {
+  float best = best_init;
+  int32_t lab = kNoise;
+  for (int col = 0; col < 5; ++col) {
+    WinColumn w;
+    load_column(cell_start, cell_flags, cell_box, row[2 * col], row[2 * col + 1], w);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (q >= w.cnt || !(w.flags[q] & kCellHasCore)) continue;
+      if (point_box_d2(x, y, w.box[q]) > eps_sq) continue;
+      const int32_t cs = w.start[q], ce = w.start[q + 1];
+      if (ce - cs > kLaneScan) {
+        deferred = true;
+        continue;
+      }
+      for (int32_t j = cs; j < ce; ++j) {
+        if (!is_core[j]) continue;
+        const float ex = xs[j] - x, ey = ys[j] - y;
+        const float d2 = ex * ex + ey * ey;
+        if (d2 <= eps_sq && d2 < best) {
+          best = d2;
+          lab = cell_cluster[w.j0 + q];
+        }
+      }
+    }
+  }
+  return lab;
+}
+
+// The same search by the whole wave for the point (xi, yi): each candidate
+// cell is scanned 64 points per step; a step's smallest accepted d2 replaces
+// the best when strictly smaller, so the first strict minimum in scan order
+// wins as in the reference's sequential loop (cores of one cell share a
+// label, so the order inside a cell does not matter).
+__device__ int32_t wave_border(const float* __restrict__ xs, const float* __restrict__ ys,
+                               const uint8_t* __restrict__ is_core, float xi, float yi,
+                               const int32_t* __restrict__ row,
+                               const int32_t* __restrict__ cell_start,
+                               const uint32_t* __restrict__ cell_flags,
+                               const uint4* __restrict__ cell_box,
+                               const int32_t* __restrict__ cell_cluster, float eps_sq,
+                               float best_init) {
+  const int lane = threadIdx.x & 63;
+  float best = best_init;
+  int32_t lab = kNoise;
+  for (int col = 0; col < 5; ++col) {
+    const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
+    for (int32_t nc = j0; nc < j1; ++nc) {
+      if (!(cell_flags[nc] & kCellHasCore)) continue;
+      if (point_box_d2(xi, yi, load_box(cell_box, nc)) > eps_sq) continue;
+      const int32_t cs = cell_start[nc], ce = cell_start[nc + 1];
+      for (int32_t base = cs; base < ce; base += 64) {
+        const int32_t j = base + lane;
+        uint32_t m = 0xFFFFFFFFu;
+        if (j < ce && is_core[j]) {
+          const float ex = xs[j] - xi, ey = ys[j] - yi;
+          const float d2 = ex * ex + ey * ey;
+          if (d2 <= eps_sq && d2 < best) m = __float_as_uint(d2);
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) m = min(m, static_cast<uint32_t>(__shfl_xor(m, d, 64)));
+        if (m != 0xFFFFFFFFu) {
+          best = __uint_as_float(m);
+          lab = cell_cluster[nc];
+        }
+      }
+    }
+  }
+  return lab;
+}
+
+// Border assignment (dbscan_grid.hpp:415-446) for the cells that hold a
+// non-core point: one lane per cell searches for each of its non-core points
+// in turn and writes the label to the point's input position.  Points whose
+// search needs a big cell, and cells of more than 64 points, are searched by
+// the whole wave.
+__global__ __launch_bounds__(256) void k_border(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const uint32_t* __restrict__ idx_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b,
-    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
-    const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
-    const uint32_t* __restrict__ cell_flags, const uint4* __restrict__ cell_box,
-    const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
-    int32_t* __restrict__ labels, int64_t n, float eps_sq, float best_init) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ cell_start,
+    const int32_t* __restrict__ ngh, const uint32_t* __restrict__ cell_flags,
+    const uint4* __restrict__ cell_box, const uint8_t* __restrict__ is_core,
+    const int32_t* __restrict__ cell_cluster, int32_t* __restrict__ labels, int64_t n,
+    float eps_sq, float best_init) {
+  const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
-  const bool valid = i < n;
-  int32_t c = 0, lab = kNoise;
-  bool need = false;
-  float x = 0.f, y = 0.f;
-  if (valid) {
-    c = point_cell[i];
-    if (is_core[i]) {
-      lab = cell_cluster[c];
-    } else {
-      need = true;
-      x = xs[i];
-      y = ys[i];
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+  const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
+  for (int64_t base = wave * 64; base < C; base += nwaves * 64) {
+    const int32_t c = static_cast<int32_t>(base + lane);
+    const bool valid = base + lane < C;
+    int32_t s = 0, e = 0;
+    bool has = false;
+    if (valid && (cell_flags[c] & kCellHasNonCore)) {
+      has = true;
+      s = cell_start[c];
+      e = cell_start[c + 1];
     }
-  }
-  uint64_t todo = __ballot(need);
-  while (todo) {
-    const int src = lowest_lane(todo);
-    todo &= todo - 1ull;
-    const int32_t ci = bcast(c, src);
-    const float xi = bcastf(x, src), yi = bcastf(y, src);
-    float best_d2 = best_init;
-    int32_t best_lab = kNoise;
-    const int32_t* row = ngh + static_cast<int64_t>(ci) * 10;
-    for (int col = 0; col < 5; ++col) {
-      const int32_t j0 = row[2 * col], j1 = row[2 * col + 1];
-      if (j0 >= j1) continue;
-      int32_t run = cell_start[j0];
-      for (int32_t nc = j0; nc < j1; ++nc) {
-        const int32_t cs = cell_start[nc], ce = cell_start[nc + 1];
-        if (ce - cs <= kBigCell) continue;
-        best_in_range(xs, ys, is_core, point_cell, cell_cluster, run, cs, xi, yi, eps_sq,
-                      best_d2, best_lab);
-        run = ce;
-        if (!(cell_flags[nc] & 1u)) continue;
-        if (point_box_d2(xi, yi, load_box(cell_box, nc)) > eps_sq) continue;
-        best_in_range(xs, ys, is_core, point_cell, cell_cluster, cs, ce, xi, yi, eps_sq,
-                      best_d2, best_lab);
+    const bool whole_wave = has && e - s > 64;
+    uint64_t pending = 0;
+    if (has && !whole_wave) {
+      const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+      for (int32_t p = s; p < e; ++p) {
+        if (is_core[p]) continue;
+        bool deferred = false;
+        const int32_t lab = lane_border(xs, ys, is_core, xs[p], ys[p], row, cell_start, cell_flags,
+                                        cell_box, cell_cluster, eps_sq, best_init, deferred);
+        if (deferred) {
+          pending |= 1ull << (p - s);
+        } else {
+          const uint32_t dst = idx[p];
+          if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
+        }
       }
-      best_in_range(xs, ys, is_core, point_cell, cell_cluster, run, cell_start[j1], xi, yi,
-                    eps_sq, best_d2, best_lab);
     }
-    if (lane == src) lab = best_lab;
+    uint64_t todo = __ballot(pending != 0 || whole_wave);
+    while (todo) {
+      const int src = lowest_lane(todo);
+      todo &= todo - 1ull;
+      const int32_t cc = bcast(c, src), cs = bcast(s, src), ce = bcast(e, src);
+      const uint64_t pm = bcast64(pending, src);
+      const bool all = ce - cs > 64;
+      const int32_t* row = ngh + static_cast<int64_t>(cc) * 10;
+      for (int32_t p = cs; p < ce; ++p) {
+        if (all ? is_core[p] != 0 : !((pm >> (p - cs)) & 1ull)) continue;
+        const int32_t lab = wave_border(xs, ys, is_core, xs[p], ys[p], row, cell_start,
+                                        cell_flags, cell_box, cell_cluster, eps_sq, best_init);
+        if (lane == src) {
+          const uint32_t dst = idx[p];
+          if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
+        }
+      }
+    }
   }
-  if (!valid) return;
-  const uint32_t dst = idx[i];
-  if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
 }
 
 // ---------------------------------------------------------------------------
@@ -1177,10 +1483,13 @@ enum KernelSlot : int {
   kKNgh,
   kKCore,
   kKUnion,
+  kKFlatten0,
+  kKUnion1,
   kKFlatten,
   kKRank,
   kKCluster,
   kKLabel,
+  kKBorder,
   kNumKernels
 };
 static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header");
@@ -1188,12 +1497,12 @@ static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header
 const char* const kKernelNames[kNumKernels] = {
     "k_bounds", "k_hist", "k_sort_pass[0]", "k_sort_pass[1]", "k_sort_pass[2]",
     "k_sort_pass[3]", "k_sort_pass[4]", "k_sort_pass[5]", "k_sort_pass[6]", "k_sort_pass[7]",
-    "k_cells", "k_bbox", "k_ngh", "k_core", "k_union", "k_flatten", "k_rank", "k_cluster", "k_label"};
+    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_flatten[3x3]", "k_union[rest]", "k_flatten", "k_rank", "k_cluster", "k_label", "k_border"};
 
 // First kernel of each reference stage (stages 7-9 and 9-10 share kernels:
 // core labels, border assignment and unpermute are one fused kernel).
 constexpr int kStageFirstKernel[DBSCAN_HIP_NUM_STAGES + 1] = {
-    kKBounds, kKSort0, kKCells, kKNgh, kKCore, kKUnion, kKRank, kKLabel, kKLabel,
+    kKBounds, kKSort0, kKCells, kKNgh, kKCore, kKUnion, kKRank, kKLabel, kKBorder,
     kNumKernels, kNumKernels};
 
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
@@ -1287,7 +1596,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
   const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
-  const int g_union = grid_for(n, 4 * 4, 16384);  // one wave per cell, up to 64k waves
+  const int g_union = grid_for(n * kUnionSlots, 256, 16384);
+  const int g_cellwaves = grid_for(n, 256, 8192);  // one lane per cell, grid-stride
 
   // Stage 1: cell keys (bounds + digit histograms).
   if ((e = before(kKBounds)) != hipSuccess) return fail(e);
@@ -1318,7 +1628,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
                                                cell_start, cell_key, cell_flags, cell_box, n,
                                                inv_r);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
-  k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_box, n);
+  k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
+                                      cell_box, cell_flags, is_core, n, ms);
 
   // Stage 4: window columns.
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
@@ -1326,13 +1637,18 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   // Stage 5: core marking.
   if ((e = before(kKCore)) != hipSuccess) return fail(e);
-  k_core<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start, ngh,
-                                      cell_box, is_core, cell_flags, n, ms, eps_sq);
+  k_core_sparse<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh,
+                                                 cell_box, is_core, cell_flags, ms, eps_sq);
 
   // Stage 6: union-find.
   if ((e = before(kKUnion)) != hipSuccess) return fail(e);
-  k_union<<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                       cell_flags, cell_box, is_core, uf, eps_sq);
+  k_union<0><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
+                                          cell_flags, cell_box, cell_key, is_core, uf, eps_sq);
+  if ((e = before(kKFlatten0)) != hipSuccess) return fail(e);
+  k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
+  if ((e = before(kKUnion1)) != hipSuccess) return fail(e);
+  k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
+                                          cell_flags, cell_box, cell_key, is_core, uf, eps_sq);
   if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
 
@@ -1344,9 +1660,12 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   // Stages 8-10: core labels, border assignment and unpermute, fused.
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  k_label<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, point_cell,
-                                       cell_start, ngh, cell_flags, cell_box, is_core, cell_cluster,
-                                       labels, n, eps_sq, best_init);
+  k_label<<<g_point, 256, 0, stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster,
+                                       labels, n);
+  if ((e = before(kKBorder)) != hipSuccess) return fail(e);
+  k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
+                                            ngh, cell_flags, cell_box, is_core, cell_cluster,
+                                            labels, n, eps_sq, best_init);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
