@@ -37,6 +37,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -47,8 +48,11 @@
 namespace {
 
 constexpr int32_t kNoise = -1;
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
+// Radix digits are at most kRadixBitsMax wide.  A key of b bits is sorted in
+// P = ceil(b / kRadixBitsMax) passes of ceil(b / P) bits each, both chosen on
+// the device once the bounds are known (26-bit keys: 3 passes of 9, 9, 8).
+constexpr int kRadixBitsMax = 8;
+constexpr int kRadixMax = 1 << kRadixBitsMax;
 constexpr int kMaxPasses = 8;           // 64-bit keys at most
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 12;
@@ -67,7 +71,8 @@ constexpr int kMaxReduceBlocks = 512;
 constexpr uint32_t kSpinLimit = 1u << 26;         // bounded waits; see ErrFlag
 constexpr uint64_t kCountMask = (1ull << 48) - 1;
 
-static_assert(kSortThreads == kRadix, "one thread per digit in k_sort_pass");
+constexpr int kDigitsPerThread = kRadixMax / kSortThreads;
+static_assert(kDigitsPerThread * kSortThreads == kRadixMax, "digits split evenly over threads");
 
 // Error bits left in Params::error when a bounded wait gives up (never
 // expected; checked by the tests through the debug layout).
@@ -88,7 +93,7 @@ struct Params {
   int32_t pad0;
   // Everything below is cleared by k_bounds at the start of every call.
   uint32_t tile_counter[16];
-  uint32_t hist[kMaxPasses][kRadix];
+  uint32_t hist[kMaxPasses][kRadixMax];
 };
 constexpr size_t kParamsZeroBegin = offsetof(Params, tile_counter);
 
@@ -105,8 +110,9 @@ __device__ __forceinline__ int32_t cell_coord(float v) {
 
 struct KeyMap {
   uint32_t kxmin, kymin;  // subtracted (mod 2^32) from kx, ky
-  uint32_t by;            // bit width of the y part
+  uint32_t bx, by;        // bit widths of the x and y parts
   int npasses;            // radix passes covering bx + by bits (>= 1)
+  uint32_t dbits;         // digit width of every pass
 };
 
 __device__ __forceinline__ uint32_t bit_width(uint32_t v) {
@@ -118,15 +124,22 @@ __device__ __forceinline__ KeyMap make_keymap(int32_t kxmin, int32_t kxmax, int3
   KeyMap m;
   m.kxmin = static_cast<uint32_t>(kxmin);
   m.kymin = static_cast<uint32_t>(kymin);
-  const uint32_t bx = bit_width(static_cast<uint32_t>(kxmax) - m.kxmin);
+  m.bx = bit_width(static_cast<uint32_t>(kxmax) - m.kxmin);
   m.by = bit_width(static_cast<uint32_t>(kymax) - m.kymin);
-  const int nbits = static_cast<int>(bx + m.by);
-  m.npasses = nbits ? (nbits + kRadixBits - 1) / kRadixBits : 1;
+  const int nbits = static_cast<int>(m.bx + m.by);
+  m.npasses = nbits ? (nbits + kRadixBitsMax - 1) / kRadixBitsMax : 1;
+  m.dbits = nbits ? static_cast<uint32_t>((nbits + m.npasses - 1) / m.npasses) : 1u;
   return m;
 }
 
 __device__ __forceinline__ KeyMap load_keymap(const Params* p) {
   return make_keymap(p->kxmin, p->kxmax, p->kymin, p->kymax);
+}
+
+// Column index: for x' < 2^bx <= max_cols, col_range[x'] = (first cell, end
+// cell) of column x' in cell order (both 0 for an empty column).
+__device__ __forceinline__ bool has_col_index(const KeyMap& m, int64_t max_cols) {
+  return m.bx < 32 && (1ll << m.bx) <= max_cols;
 }
 
 // Relative cell coordinates (x', y') = (kx - kxmin, ky - kymin) of a point.
@@ -141,6 +154,12 @@ __device__ __forceinline__ uint64_t sort_key(float x, float y, float inv_r, cons
   uint32_t xr, yr;
   rel_cell(x, y, inv_r, m, xr, yr);
   return (static_cast<uint64_t>(xr) << m.by) | yr;
+}
+
+// Digit of radix pass `pass`.
+__device__ __forceinline__ uint32_t key_digit(uint64_t key, int pass, const KeyMap& m) {
+  return static_cast<uint32_t>(key >> (m.dbits * static_cast<uint32_t>(pass))) &
+         ((1u << m.dbits) - 1u);
 }
 
 // Cell key in internal order: (x' << 32 | y').
@@ -301,11 +320,27 @@ __global__ __launch_bounds__(256) void k_bounds(const float2* __restrict__ X, in
   for (int64_t z = gtid; z < zero16; z += stride) zero[z] = make_uint4(0, 0, 0, 0);
 
   int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-  for (int64_t i = gtid; i < n; i += stride) {
-    const float2 p = X[i];
-    const int32_t kx = cell_coord(p.x * inv_r), ky = cell_coord(p.y * inv_r);
+  auto take = [&](float x, float y) {
+    const int32_t kx = cell_coord(x * inv_r), ky = cell_coord(y * inv_r);
     v[0] = min(v[0], kx); v[1] = max(v[1], kx);
     v[2] = min(v[2], ky); v[3] = max(v[3], ky);
+  };
+  if (reinterpret_cast<uintptr_t>(X) & 15u) {  // only 8-byte aligned: one point per load
+    for (int64_t i = gtid; i < n; i += stride) take(X[i].x, X[i].y);
+  } else {
+    // two points per 16-byte load, two loads in flight per thread
+    const float4* __restrict__ X4 = reinterpret_cast<const float4*>(X);
+    const int64_t pairs = n >> 1;
+    int64_t q = gtid;
+    for (; q + stride < pairs; q += 2 * stride) {
+      const float4 a = X4[q], b = X4[q + stride];
+      take(a.x, a.y); take(a.z, a.w); take(b.x, b.y); take(b.z, b.w);
+    }
+    if (q < pairs) {
+      const float4 a = X4[q];
+      take(a.x, a.y); take(a.z, a.w);
+    }
+    if ((n & 1) && gtid == 0) take(X[n - 1].x, X[n - 1].y);
   }
   __shared__ int32_t s[4][4];
   block_minmax4(v, s);
@@ -316,9 +351,10 @@ __global__ __launch_bounds__(256) void k_bounds(const float2* __restrict__ X, in
 // histogram of every radix pass.
 __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int64_t n,
                                               float inv_r, const int4* __restrict__ partials,
-                                              int nparts, Params* __restrict__ prm) {
+                                              int nparts, Params* __restrict__ prm,
+                                              int2* __restrict__ col_range, int64_t max_cols) {
   __shared__ int32_t s_red[4][4];
-  __shared__ uint32_t s_hist[kMaxPasses][kRadix];
+  __shared__ uint32_t s_hist[kMaxPasses][kRadixMax];
   int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
   for (int i = threadIdx.x; i < nparts; i += 256) {
     const int4 q = partials[i];
@@ -333,17 +369,35 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
     prm->num_cells = 0;
     prm->error = 0;
   }
-  for (int e = threadIdx.x; e < kMaxPasses * kRadix; e += 256) (&s_hist[0][0])[e] = 0u;
+  for (int e = threadIdx.x; e < kMaxPasses * kRadixMax; e += 256) (&s_hist[0][0])[e] = 0u;
   __syncthreads();
   const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-    const float2 p = X[i];
-    const uint64_t key = sort_key(p.x, p.y, inv_r, m);
-    for (int ps = 0; ps < m.npasses; ++ps)
-      atomicAdd(&s_hist[ps][static_cast<uint32_t>(key >> (kRadixBits * ps)) & (kRadix - 1)], 1u);
+  const int64_t gtid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (has_col_index(m, max_cols))
+    for (int64_t q = gtid; q < (1ll << m.bx); q += stride) col_range[q] = make_int2(0, 0);
+  auto take = [&](float x, float y) {
+    const uint64_t key = sort_key(x, y, inv_r, m);
+    for (int ps = 0; ps < m.npasses; ++ps) atomicAdd(&s_hist[ps][key_digit(key, ps, m)], 1u);
+  };
+  if (reinterpret_cast<uintptr_t>(X) & 15u) {  // only 8-byte aligned: one point per load
+    for (int64_t i = gtid; i < n; i += stride) take(X[i].x, X[i].y);
+  } else {
+    // two points per 16-byte load, two loads in flight per thread
+    const float4* __restrict__ X4 = reinterpret_cast<const float4*>(X);
+    const int64_t pairs = n >> 1;
+    int64_t q = gtid;
+    for (; q + stride < pairs; q += 2 * stride) {
+      const float4 a = X4[q], b = X4[q + stride];
+      take(a.x, a.y); take(a.z, a.w); take(b.x, b.y); take(b.z, b.w);
+    }
+    if (q < pairs) {
+      const float4 a = X4[q];
+      take(a.x, a.y); take(a.z, a.w);
+    }
+    if ((n & 1) && gtid == 0) take(X[n - 1].x, X[n - 1].y);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < m.npasses * kRadix; e += 256) {
+  for (int e = threadIdx.x; e < m.npasses * kRadixMax; e += 256) {
     const uint32_t c = (&s_hist[0][0])[e];
     if (c) atomicAdd(&prm->hist[0][0] + e, c);
   }
@@ -364,7 +418,7 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
     uint64_t w[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      w[k] = (j - k >= 0) ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadix + digit)
+      w[k] = (j - k >= 0) ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadixMax + digit)
                           : tag_pre;
     // consume the published words in order: `used` of them, `stop` at a prefix
     int used = 0;
@@ -396,13 +450,14 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
   return excl;
 }
 
-// One stable LSD radix pass over the digit (key >> 8*pass) & 255.
+// One stable LSD radix pass over digit `pass` of the compressed cell key.
 // kFirst: read the interleaved input X and emit original indices.
 // A tile of kSortTile points is ranked in LDS (wave64 ballot matching into
 // per-wave counters, item by item, which keeps equal digits in input order),
 // offset by the global digit histogram and a look-back over earlier tiles,
 // then reordered by digit through one LDS buffer, array by array, so that runs
-// of equal digits leave as contiguous writes.
+// of equal digits leave as contiguous writes.  Thread t owns digits
+// [t*kDigitsPerThread, (t+1)*kDigitsPerThread).
 template <bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
@@ -412,23 +467,37 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31: 32-bit offsets
   const KeyMap m = load_keymap(prm);
   if (pass >= m.npasses) return;
+  constexpr int D = kDigitsPerThread;
 
   __shared__ uint32_t s_buf[kSortTile];
-  __shared__ uint8_t s_d[kSortTile];
-  __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadix];
-  __shared__ uint32_t s_goff[kRadix];
-  __shared__ uint32_t s_dstart[kRadix];
+  __shared__ uint16_t s_d[kSortTile];
+  __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadixMax];
+  __shared__ uint32_t s_goff[kRadixMax];
+  __shared__ uint32_t s_dstart[kRadixMax];
   __shared__ uint32_t s_scan[kSortThreads / 64];
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  for (int e = t; e < (kSortThreads / 64) * kRadixMax; e += kSortThreads) (&s_wcnt[0][0])[e] = 0u;
+  // global start of each owned digit: exclusive scan of the pass histogram
+  uint32_t goff[D];
+  {
+    uint32_t h[D], sum = 0;
 #pragma unroll
-  for (int w = 0; w < kSortThreads / 64; ++w) s_wcnt[w][t] = 0u;
-  uint32_t htot;
-  const uint32_t goff = block_excl_scan<kSortThreads>(prm->hist[pass][t], htot, s_scan);
+    for (int q = 0; q < D; ++q) {
+      h[q] = prm->hist[pass][t * D + q];
+      sum += h[q];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<kSortThreads>(sum, tot, s_scan);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      goff[q] = run;
+      run += h[q];
+    }
+  }
   const uint32_t tile = blockIdx.x;  // see kTileOrder
   const uint32_t base = tile * kSortTile;
   const uint32_t wbase = base + static_cast<uint32_t>(wave) * (64 * kSortItems);
-  const int shift = kRadixBits * pass;
 
   float x[kSortItems], y[kSortItems];
   uint32_t id[kSortItems], dg[kSortItems], rk[kSortItems];
@@ -448,22 +517,23 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   // all loads of the tile are in flight before the first key is computed
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k)
-    dg[k] = static_cast<uint32_t>(sort_key(x[k], y[k], inv_r, m) >> shift) & (kRadix - 1);
+  for (int k = 0; k < kSortItems; ++k) dg[k] = key_digit(sort_key(x[k], y[k], inv_r, m), pass, m);
 
   // Rank within the wave's segment, item by item (stable): lanes holding the
-  // same digit are found with 8 ballots; the lowest such lane bumps the wave's
-  // counter for that digit.
+  // same digit are found with one ballot per digit bit; the lowest such lane
+  // bumps the wave's counter for that digit.
   const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
     const bool valid = wbase + k * 64 + lane < n;
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
-      const bool bit = (dg[k] >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+    for (int b = 0; b < kRadixBitsMax; ++b) {
+      if (static_cast<uint32_t>(b) < m.dbits) {
+        const bool bit = (dg[k] >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
     }
     const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
     uint32_t old = 0;
@@ -473,26 +543,37 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   }
   __syncthreads();
 
-  // Thread t owns digit t: wave offsets, tile total, tile-local start.
-  const uint32_t c0 = s_wcnt[0][t], c1 = s_wcnt[1][t], c2 = s_wcnt[2][t], c3 = s_wcnt[3][t];
-  const uint32_t dtot = c0 + c1 + c2 + c3;
-  s_wcnt[0][t] = 0u;
-  s_wcnt[1][t] = c0;
-  s_wcnt[2][t] = c0 + c1;
-  s_wcnt[3][t] = c0 + c1 + c2;
+  // Owned digits: wave offsets, tile totals, aggregate publication.
   const uint64_t tag_agg = static_cast<uint64_t>(2 * pass + 1) << 48;
   const uint64_t tag_pre = static_cast<uint64_t>(2 * pass + 2) << 48;
-  uint64_t* my_status = status + static_cast<uint64_t>(tile) * kRadix + t;
-  st_release(my_status, (tile == 0 ? tag_pre : tag_agg) | dtot);
-  uint32_t ttot;
-  const uint32_t dstart = block_excl_scan<kSortThreads>(dtot, ttot, s_scan);
-  s_dstart[t] = dstart;
-  uint64_t excl = 0;
-  if (tile > 0) {
-    excl = digit_lookback(status, static_cast<int64_t>(tile), t, tag_agg, tag_pre, &prm->error);
-    st_release(my_status, tag_pre | (excl + dtot));
+  uint32_t dtot[D], dsum = 0;
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    const int d = t * D + q;
+    const uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
+    dtot[q] = c0 + c1 + c2 + c3;
+    s_wcnt[0][d] = 0u;
+    s_wcnt[1][d] = c0;
+    s_wcnt[2][d] = c0 + c1;
+    s_wcnt[3][d] = c0 + c1 + c2;
+    st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
+               (tile == 0 ? tag_pre : tag_agg) | dtot[q]);
+    dsum += dtot[q];
   }
-  s_goff[t] = goff + static_cast<uint32_t>(excl) - dstart;
+  uint32_t ttot;
+  uint32_t dstart = block_excl_scan<kSortThreads>(dsum, ttot, s_scan);
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    const int d = t * D + q;
+    uint64_t excl = 0;
+    if (tile > 0) {
+      excl = digit_lookback(status, static_cast<int64_t>(tile), d, tag_agg, tag_pre, &prm->error);
+      st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
+    }
+    s_dstart[d] = dstart;
+    s_goff[d] = goff[q] + static_cast<uint32_t>(excl) - dstart;
+    dstart += dtot[q];
+  }
   __syncthreads();
 
   // Tile-local position of every item (kept in rk), then the digits in
@@ -500,7 +581,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
     rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
-    if (wbase + k * 64 + lane < n) s_d[rk[k]] = static_cast<uint8_t>(dg[k]);
+    if (wbase + k * 64 + lane < n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
   }
   const int tile_n = static_cast<int>(min(static_cast<uint32_t>(kSortTile), n - base));
   // x, y and index in turn through the one buffer
@@ -526,35 +607,54 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 
 // Cell runs over the sorted points (dbscan_grid.hpp:206-227): flags where the
 // key changes, block scan plus look-back, then point->cell, cell starts and
-// cell keys.  Blocked layout: thread t owns kScanItems consecutive points.
+// cell keys.  Blocked layout: thread t owns kScanItems consecutive points,
+// read and written with 16-byte vector accesses.
 __global__ __launch_bounds__(kScanThreads) void k_cells(
     const float* __restrict__ xs_a, const float* __restrict__ ys_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b, Params* __restrict__ prm,
     uint64_t* __restrict__ status, int32_t* __restrict__ point_cell,
     int32_t* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
-    uint32_t* __restrict__ cell_flags, uint4* __restrict__ cell_box, int64_t n, float inv_r) {
+    uint32_t* __restrict__ cell_flags, uint4* __restrict__ cell_box, int2* __restrict__ col_range,
+    int64_t max_cols, int64_t n64, float inv_r) {
+  static_assert(kScanItems == 8, "two float4 per thread");
   __shared__ uint64_t s_last[kScanThreads];
   __shared__ uint32_t s_scan[kScanThreads / 64];
   __shared__ uint64_t s_prefix;
+  const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31
   const KeyMap m = load_keymap(prm);
   const bool in_a = (m.npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const int t = threadIdx.x;
   const uint32_t tile = blockIdx.x;  // see kTileOrder
-  const int64_t base = static_cast<int64_t>(tile) * kScanTile;
-  const int64_t e0 = base + static_cast<int64_t>(t) * kScanItems;
+  const uint32_t base = tile * kScanTile;
+  const uint32_t e0 = base + static_cast<uint32_t>(t) * kScanItems;
+  const bool whole = e0 + kScanItems <= n;
 
+  float px[kScanItems], py[kScanItems];
+  if (whole) {
+    const float4 x0 = *reinterpret_cast<const float4*>(xs + e0);
+    const float4 x1 = *reinterpret_cast<const float4*>(xs + e0 + 4);
+    const float4 y0 = *reinterpret_cast<const float4*>(ys + e0);
+    const float4 y1 = *reinterpret_cast<const float4*>(ys + e0 + 4);
+    px[0] = x0.x; px[1] = x0.y; px[2] = x0.z; px[3] = x0.w;
+    px[4] = x1.x; px[5] = x1.y; px[6] = x1.z; px[7] = x1.w;
+    py[0] = y0.x; py[1] = y0.y; py[2] = y0.z; py[3] = y0.w;
+    py[4] = y1.x; py[5] = y1.y; py[6] = y1.z; py[7] = y1.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+      const uint32_t ec = e0 + j < n ? e0 + j : n - 1;  // clamped: defined, never used
+      px[j] = xs[ec];
+      py[j] = ys[ec];
+    }
+  }
   uint64_t key[kScanItems];
-  uint32_t flags = 0, cnt = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    key[j] = 0;
-    if (e0 + j < n) {
-      uint32_t xr, yr;
-      rel_cell(xs[e0 + j], ys[e0 + j], inv_r, m, xr, yr);
-      key[j] = cell_key64(xr, yr);
-    }
+    uint32_t xr, yr;
+    rel_cell(px[j], py[j], inv_r, m, xr, yr);
+    key[j] = cell_key64(xr, yr);
   }
   s_last[t] = key[kScanItems - 1];
   __syncthreads();
@@ -569,42 +669,58 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     prev = cell_key64(xr, yr);
     have_prev = true;
   }
+  uint32_t flags = 0, newcol = 0, cnt = 0;
+  uint32_t prev_x[kScanItems];
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    if (e0 + j < n) {
-      const bool f = !have_prev || key[j] != prev;
-      flags |= static_cast<uint32_t>(f) << j;
-      cnt += f;
-      prev = key[j];
-      have_prev = true;
-    }
+    const bool f = e0 + j < n && (!have_prev || key[j] != prev);
+    flags |= static_cast<uint32_t>(f) << j;
+    newcol |= static_cast<uint32_t>(f && (!have_prev || (key[j] >> 32) != (prev >> 32))) << j;
+    prev_x[j] = have_prev ? static_cast<uint32_t>(prev >> 32) : 0u;
+    cnt += f;
+    prev = key[j];
+    have_prev = true;
   }
+  const bool colidx = has_col_index(m, max_cols);
   uint32_t ttot;
   const uint32_t texcl = block_excl_scan<kScanThreads>(cnt, ttot, s_scan);
   if (t == 0) s_prefix = lookback_single(status, tile, ttot, &prm->error);
   __syncthreads();
-  int64_t c = static_cast<int64_t>(s_prefix) + texcl - 1;
+  int32_t c = static_cast<int32_t>(s_prefix + texcl) - 1;
+  int32_t pc[kScanItems];
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    const int64_t e = e0 + j;
-    if (e < n) {
-      if (c + ((flags >> j) & 1u) >= n) {
-        atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
-        break;
-      }
-      if ((flags >> j) & 1u) {
-        ++c;
-        cell_start[c] = static_cast<int32_t>(e);
+    if ((flags >> j) & 1u) {
+      ++c;
+      if (static_cast<uint32_t>(c) < n) {
+        cell_start[c] = static_cast<int32_t>(e0 + j);
         cell_key[c] = key[j];
         cell_flags[c] = 0u;
         cell_box[c] = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);  // empty box for k_bbox
-      }
-      point_cell[e] = static_cast<int32_t>(c);
-      if (e == n - 1) {
-        prm->num_cells = static_cast<int32_t>(c + 1);
-        cell_start[c + 1] = static_cast<int32_t>(n);
+        if (colidx && ((newcol >> j) & 1u)) {
+          col_range[key[j] >> 32].x = c;
+          if (c > 0) col_range[prev_x[j]].y = c;
+        }
+      } else {
+        atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
       }
     }
+    pc[j] = c;
+  }
+  if (whole) {
+    *reinterpret_cast<int4*>(point_cell + e0) = make_int4(pc[0], pc[1], pc[2], pc[3]);
+    *reinterpret_cast<int4*>(point_cell + e0 + 4) = make_int4(pc[4], pc[5], pc[6], pc[7]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+      if (e0 + j < n) point_cell[e0 + j] = pc[j];
+  }
+  if (e0 <= n - 1 && n - 1 < e0 + kScanItems) {  // this thread holds the last point
+    const int j = static_cast<int>((n - 1) - e0);
+    const int32_t last = pc[j];
+    prm->num_cells = last + 1;
+    cell_start[last + 1] = static_cast<int32_t>(n);
+    if (colidx) col_range[key[j] >> 32].y = last + 1;
   }
 }
 
@@ -623,48 +739,73 @@ __device__ __forceinline__ int32_t lower_bound_u64(const uint64_t* __restrict__ 
 // cells with y' in [y'-2, y'+2] are a contiguous run [j0, j1) of cells, so
 // the 25-cell window of the reference (dbscan_grid.hpp:239-252) is 5 runs,
 // and their order (dx-major, then ky ascending) is the reference's window
-// offset order off = (dx+2)*5 + (dy+2).
+// offset order off = (dx+2)*5 + (dy+2).  With a column index the searches
+// run inside one column; otherwise over all cells.
 __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
                                              const uint64_t* __restrict__ cell_key,
+                                             const int2* __restrict__ col_range, int64_t max_cols,
                                              int32_t* __restrict__ ngh,
                                              int32_t* __restrict__ cell_ref,
                                              int32_t* __restrict__ uf) {
   const int32_t C = prm->num_cells;
+  const KeyMap m = load_keymap(prm);
+  const bool colidx = has_col_index(m, max_cols);
+  const int64_t ncols = colidx ? (1ll << m.bx) : 0;
   const bool straddle = prm->kymin < 0 && prm->kymax >= 0;
   const uint32_t yr0 = 0u - static_cast<uint32_t>(prm->kymin);  // y' of ky == 0
-  for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
-       c += static_cast<int64_t>(gridDim.x) * 256) {
+  // cells of column x (x < 2^32): [lo, hi)
+  auto column = [&](int64_t x, int32_t& lo, int32_t& hi) {
+    if (colidx) {
+      if (x >= ncols) {
+        lo = hi = 0;
+      } else {
+        const int2 r = col_range[x];
+        lo = r.x;
+        hi = r.y;
+      }
+    } else {
+      lo = lower_bound_u64(cell_key, C, static_cast<uint64_t>(x) << 32);
+      hi = x == 0xFFFFFFFFll ? C
+                             : lower_bound_u64(cell_key, C, static_cast<uint64_t>(x + 1) << 32);
+    }
+  };
+  for (int64_t c64 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c64 < C;
+       c64 += static_cast<int64_t>(gridDim.x) * 256) {
+    const int32_t c = static_cast<int32_t>(c64);
     const uint64_t key = cell_key[c];
     const uint32_t xr = static_cast<uint32_t>(key >> 32), yr = static_cast<uint32_t>(key);
     const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
     const uint32_t yhi = yr <= 0xFFFFFFFDu ? yr + 2u : 0xFFFFFFFFu;
-    int32_t* row = ngh + c * 10;
+    int32_t* row = ngh + c64 * 10;
+    int32_t own_lo = 0, own_hi = 0;
 #pragma unroll
-    for (int dx = -2; dx <= 2; ++dx) {
-      const int64_t cx = static_cast<int64_t>(xr) + dx;
+    for (int k = 0; k < 5; ++k) {
+      const int64_t cx = static_cast<int64_t>(xr) + (k - 2);
       int32_t j0 = 0, j1 = 0;
       if (cx >= 0 && cx <= 0xFFFFFFFFll) {
+        int32_t lo, hi;
+        column(cx, lo, hi);
+        if (k == 2) {
+          own_lo = lo;
+          own_hi = hi;
+        }
         const uint64_t colbase = static_cast<uint64_t>(cx) << 32;
-        j0 = lower_bound_u64(cell_key, C, colbase | ylo);
+        j0 = lo + lower_bound_u64(cell_key + lo, hi - lo, colbase | ylo);
         j1 = j0;
-        while (j1 < C && cell_key[j1] <= (colbase | yhi)) ++j1;
+        while (j1 < hi && cell_key[j1] <= (colbase | yhi)) ++j1;
       }
-      row[2 * (dx + 2)] = j0;
-      row[2 * (dx + 2) + 1] = j1;
+      row[2 * k] = j0;
+      row[2 * k + 1] = j1;
     }
-    int32_t r = static_cast<int32_t>(c);
+    int32_t r = c;
     if (straddle) {
-      const uint64_t colbase = static_cast<uint64_t>(xr) << 32;
-      const int32_t col_start = lower_bound_u64(cell_key, C, colbase);
-      const int32_t col_end =
-          xr == 0xFFFFFFFFu ? C : lower_bound_u64(cell_key, C, static_cast<uint64_t>(xr + 1u) << 32);
-      const int32_t split = lower_bound_u64(cell_key, C, colbase | yr0);
-      r = (yr >= yr0) ? col_start + (static_cast<int32_t>(c) - split)
-                      : static_cast<int32_t>(c) + (col_end - split);
+      const int32_t split = own_lo + lower_bound_u64(cell_key + own_lo, own_hi - own_lo,
+                                                     (static_cast<uint64_t>(xr) << 32) | yr0);
+      r = (yr >= yr0) ? own_lo + (c - split) : c + (own_hi - split);
     }
     if (r < 0 || r >= C) {
       atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
-      r = static_cast<int32_t>(c);
+      r = c;
     }
     cell_ref[c] = r;
     uf[r] = r;
@@ -802,7 +943,67 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
 
 // Cells up to this many points are scanned by a single lane; larger cells are
 // handed to the whole wave.
-constexpr int32_t kLaneScan = 48;
+constexpr int32_t kLaneScan = 256;
+
+// Counts the points of [cs, ce) within eps of (x, y) for one lane, four
+// candidates per step, until `total` reaches ms.
+__device__ __forceinline__ int32_t lane_count_range(const float* __restrict__ xs,
+                                                    const float* __restrict__ ys, int32_t cs,
+                                                    int32_t ce, float x, float y, float eps_sq,
+                                                    int32_t ms, int32_t total) {
+  int32_t j = cs;
+  for (; j + 4 <= ce && total < ms; j += 4) {
+    float px[4], py[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      px[u] = xs[j + u];
+      py[u] = ys[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float ex = px[u] - x, ey = py[u] - y;
+      total += (ex * ex + ey * ey <= eps_sq) ? 1 : 0;
+    }
+  }
+  for (; j < ce && total < ms; ++j) {
+    const float ex = xs[j] - x, ey = ys[j] - y;
+    total += (ex * ex + ey * ey <= eps_sq) ? 1 : 0;
+  }
+  return total;
+}
+
+// Smallest d2 <= eps_sq from (x, y) to a core point of [cs, ce), as float
+// bits (0xFFFFFFFF if none), for one lane, four candidates per step.
+__device__ __forceinline__ uint32_t lane_min_core_d2(const float* __restrict__ xs,
+                                                     const float* __restrict__ ys,
+                                                     const uint8_t* __restrict__ is_core,
+                                                     int32_t cs, int32_t ce, float x, float y,
+                                                     float eps_sq) {
+  uint32_t m = 0xFFFFFFFFu;
+  int32_t j = cs;
+  for (; j + 4 <= ce; j += 4) {
+    float px[4], py[4];
+    uint8_t pc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      px[u] = xs[j + u];
+      py[u] = ys[j + u];
+      pc[u] = is_core[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float ex = px[u] - x, ey = py[u] - y;
+      const float d2 = ex * ex + ey * ey;
+      if (pc[u] && d2 <= eps_sq) m = min(m, __float_as_uint(d2));
+    }
+  }
+  for (; j < ce; ++j) {
+    const float ex = xs[j] - x, ey = ys[j] - y;
+    const float d2 = ex * ex + ey * ey;
+    if (is_core[j] && d2 <= eps_sq) m = min(m, __float_as_uint(d2));
+  }
+  return m;
+}
 // A cell pair whose point-count product is at most this is tested by one lane.
 constexpr int32_t kLanePairs = 1024;
 
@@ -892,10 +1093,7 @@ __device__ __forceinline__ int32_t lane_core_count(
       if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq)) {
         total += ce - cs;
       } else if (ce - cs <= kLaneScan) {
-        for (int32_t j = cs; j < ce; ++j) {
-          const float ex = xs[j] - x, ey = ys[j] - y;
-          if (ex * ex + ey * ey <= eps_sq && ++total >= min_samples) break;
-        }
+        total = lane_count_range(xs, ys, cs, ce, x, y, eps_sq, min_samples, total);
       } else {
         deferred = true;
       }
@@ -1070,7 +1268,25 @@ __device__ __forceinline__ bool core_pair_within_lane(const float* __restrict__ 
     if (!is_core[a]) continue;
     const float xa = xs[a], ya = ys[a];
     if (point_box_d2(xa, ya, box_b) > eps_sq) continue;
-    for (int32_t b = sb; b < eb; ++b) {
+    int32_t b = sb;
+    for (; b + 4 <= eb; b += 4) {
+      float px[4], py[4];
+      uint8_t pc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        px[u] = xs[b + u];
+        py[u] = ys[b + u];
+        pc[u] = is_core[b + u];
+      }
+      bool hit = false;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float ex = px[u] - xa, ey = py[u] - ya;
+        hit |= pc[u] && ex * ex + ey * ey <= eps_sq;
+      }
+      if (hit) return true;
+    }
+    for (; b < eb; ++b) {
       if (!is_core[b]) continue;
       const float ex = xs[b] - xa, ey = ys[b] - ya;
       if (ex * ex + ey * ey <= eps_sq) return true;
@@ -1083,13 +1299,31 @@ __device__ __forceinline__ bool core_pair_within_lane(const float* __restrict__ 
 // lane per (cell c, forward neighbour slot): each unordered window pair is
 // visited once, from its smaller internal index (the rest of c's column above
 // c, and the two columns to the right; at most 12 slots).  Pairs with a core
-// on both sides and boxes within eps are tested by the lane when small, and
-// by the whole wave otherwise.
+// on both sides and boxes within eps are tested by the lane when small, and by
+// the whole wave otherwise.
 // Two launches split the pairs: kPhase 0 takes the 3x3 neighbours (dx <= 1,
-// |dy| <= 1), which join almost every dense region; after a flatten, kPhase 1
-// takes the rest and first skips pairs whose cells already share a root.
-// The union is order-independent, so the split cannot change a label.
+// |dy| <= 1; only slots 0-5 can hold one), which join almost every dense
+// region; after a flatten, kPhase 1 takes the rest and first skips pairs whose
+// cells already share a root.  The union is order-independent, so the split
+// cannot change a label.
 constexpr int kUnionSlots = 12;
+
+// Forward neighbour `slot` of cell c from its window row, with its column
+// offset, or -1.  Slots 0-1: the next two cells of c's column; 2-6: the cells
+// of the column to the right; 7-11: the column after that.
+__device__ __forceinline__ int32_t forward_neighbour(const int32_t* __restrict__ row, int32_t c,
+                                                     int slot, int& dx) {
+  if (slot < 2) {
+    dx = 0;
+    return c + 1 + slot < row[5] ? c + 1 + slot : -1;
+  }
+  if (slot < 7) {
+    dx = 1;
+    return row[6] + (slot - 2) < row[7] ? row[6] + (slot - 2) : -1;
+  }
+  dx = 2;
+  return row[8] + (slot - 7) < row[9] ? row[8] + (slot - 7) : -1;
+}
 
 template <int kPhase>
 __global__ __launch_bounds__(256) void k_union(
@@ -1099,6 +1333,7 @@ __global__ __launch_bounds__(256) void k_union(
     const int32_t* __restrict__ cell_ref, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const uint64_t* __restrict__ cell_key,
     const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, float eps_sq) {
+  constexpr int S = kPhase == 0 ? 7 : kUnionSlots;  // slots scanned by this phase
   const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
@@ -1106,54 +1341,38 @@ __global__ __launch_bounds__(256) void k_union(
   const int lane = threadIdx.x & 63;
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
-  const int64_t total = C * kUnionSlots;
+  const int64_t total = C * S;
   for (int64_t base = wave * 64; base < total; base += nwaves * 64) {
     const int64_t t = base + lane;
     int32_t c = -1, nc = -1;
     if (t < total) {
-      c = static_cast<int32_t>(t / kUnionSlots);
-      const int slot = static_cast<int>(t - static_cast<int64_t>(c) * kUnionSlots);
+      c = static_cast<int32_t>(t / S);
+      const int slot = static_cast<int>(t - static_cast<int64_t>(c) * S);
       if (cell_flags[c] & kCellHasCore) {
-        const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-        const int32_t n0 = row[5] - (c + 1), n1 = row[7] - row[6], n2 = row[9] - row[8];
-        int dx = 0;
-        if (slot < n0) {
-          nc = c + 1 + slot;
-        } else if (slot < n0 + n1) {
-          nc = row[6] + (slot - n0);
-          dx = 1;
-        } else if (slot < n0 + n1 + n2) {
-          nc = row[8] + (slot - n0 - n1);
-          dx = 2;
-        }
+        int dx;
+        nc = forward_neighbour(ngh + static_cast<int64_t>(c) * 10, c, slot, dx);
         if (nc >= 0) {
           const int64_t dy = static_cast<int64_t>(static_cast<uint32_t>(cell_key[nc])) -
                              static_cast<int64_t>(static_cast<uint32_t>(cell_key[c]));
-          const bool near3 = dx <= 1 && dy >= -1 && dy <= 1;
-          if (near3 != (kPhase == 0)) nc = -1;
+          if ((dx <= 1 && dy >= -1 && dy <= 1) != (kPhase == 0)) nc = -1;
         }
       }
     }
-    bool cand = false, deferred = false;
-    float4 box_c = make_float4(0.f, 0.f, 0.f, 0.f), box_n = box_c;
-    int32_t sa = 0, ea = 0, sb = 0, eb = 0;
-    if (nc >= 0 && (cell_flags[nc] & kCellHasCore)) {
-      // after the first phase's flatten, uf holds roots: equal roots need no test
-      if (kPhase == 1 && uf[cell_ref[c]] == uf[cell_ref[nc]]) nc = -1;
-    }
-    if (nc >= 0 && (cell_flags[nc] & kCellHasCore)) {
-      box_c = load_box(cell_box, c);
-      box_n = load_box(cell_box, nc);
-      cand = box_box_d2(box_c, box_n) <= eps_sq;
-    }
-    if (cand) {
-      sa = cell_start[c]; ea = cell_start[c + 1];
-      sb = cell_start[nc]; eb = cell_start[nc + 1];
-      if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs) {
-        if (core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq))
-          uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
-      } else {
-        deferred = true;
+    // after the first phase's flatten, uf holds roots: equal roots need no test
+    if (nc >= 0 && !(cell_flags[nc] & kCellHasCore)) nc = -1;
+    if (kPhase == 1 && nc >= 0 && uf[cell_ref[c]] == uf[cell_ref[nc]]) nc = -1;
+    bool deferred = false;
+    if (nc >= 0) {
+      const float4 box_c = load_box(cell_box, c), box_n = load_box(cell_box, nc);
+      if (box_box_d2(box_c, box_n) <= eps_sq) {
+        const int32_t sa = cell_start[c], ea = cell_start[c + 1];
+        const int32_t sb = cell_start[nc], eb = cell_start[nc + 1];
+        if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs) {
+          if (core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq))
+            uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
+        } else {
+          deferred = true;
+        }
       }
     }
     uint64_t todo = __ballot(deferred);
@@ -1163,9 +1382,9 @@ __global__ __launch_bounds__(256) void k_union(
       const int32_t cw = bcast(c, src), nw = bcast(nc, src);
       const int32_t rc = cell_ref[cw], rn = cell_ref[nw];
       if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
-      const float4 bc = load_box(cell_box, cw), bn = load_box(cell_box, nw);
-      const bool hit = core_pair_within(xs, ys, is_core, bcast(sa, src), bcast(ea, src), bc,
-                                        bcast(sb, src), bcast(eb, src), bn, eps_sq);
+      const bool hit = core_pair_within(xs, ys, is_core, cell_start[cw], cell_start[cw + 1],
+                                        load_box(cell_box, cw), cell_start[nw],
+                                        cell_start[nw + 1], load_box(cell_box, nw), eps_sq);
       if (hit && lane == 0) uf_unite(uf, rc, rn, &prm->error);
     }
   }
@@ -1275,14 +1494,13 @@ __device__ __forceinline__ int32_t lane_border(
         deferred = true;
         continue;
       }
-      for (int32_t j = cs; j < ce; ++j) {
-        if (!is_core[j]) continue;
-        const float ex = xs[j] - x, ey = ys[j] - y;
-        const float d2 = ex * ex + ey * ey;
-        if (d2 <= eps_sq && d2 < best) {
-          best = d2;
-          lab = cell_cluster[w.j0 + q];
-        }
+      // the cell's smallest accepted d2 replaces the best only when strictly
+      // smaller: the sequential first-strict-minimum rule, since all cores of
+      // a cell share its label
+      const uint32_t cm = lane_min_core_d2(xs, ys, is_core, cs, ce, x, y, eps_sq);
+      if (cm != 0xFFFFFFFFu && __uint_as_float(cm) < best) {
+        best = __uint_as_float(cm);
+        lab = cell_cluster[w.j0 + q];
       }
     }
   }
@@ -1410,7 +1628,8 @@ struct Layout {
   size_t partials;
   size_t xs_a, ys_a, idx_a, xs_b, ys_b, idx_b;
   size_t point_cell, is_core, cell_start, cell_key, cell_flags, cell_box, ngh, cell_ref, uf, rank,
-      cell_cluster;
+      cell_cluster, col_range;
+  int64_t max_cols;
   size_t total;
 };
 
@@ -1428,7 +1647,7 @@ Layout make_layout(int64_t n) {
     return at;
   };
   L.params = take(sizeof(Params));
-  L.sort_status = take(sort_tiles * kRadix * sizeof(uint64_t));
+  L.sort_status = take(sort_tiles * kRadixMax * sizeof(uint64_t));
   L.cell_status = take(scan_tiles * sizeof(uint64_t));
   L.rank_status = take(scan_tiles * sizeof(uint64_t));
   L.zero_end = off;
@@ -1450,6 +1669,8 @@ Layout make_layout(int64_t n) {
   L.uf = take(N * 4);
   L.rank = take(N * 4);
   L.cell_cluster = take(N * 4);
+  L.max_cols = static_cast<int64_t>(std::min<size_t>(size_t(1) << 22, 4 * N + 64));
+  L.col_range = take(static_cast<size_t>(L.max_cols) * sizeof(int2));
   L.total = off;
   return L;
 }
@@ -1596,8 +1817,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
   const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
-  const int g_union = grid_for(n * kUnionSlots, 256, 16384);
   const int g_cellwaves = grid_for(n, 256, 8192);  // one lane per cell, grid-stride
+  const int g_union = grid_for(n * 7, 256, 16384);
 
   // Stage 1: cell keys (bounds + digit histograms).
   if ((e = before(kKBounds)) != hipSuccess) return fail(e);
@@ -1605,7 +1826,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   k_bounds<<<g_reduce, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials),
                                          at<uint4>(ws, kParamsZeroBegin), zero16);
   if ((e = before(kKHist)) != hipSuccess) return fail(e);
-  k_hist<<<g_hist, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm);
+  k_hist<<<g_hist, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm,
+                                     at<int2>(ws, L.col_range), L.max_cols);
 
   // Stage 2: radix passes (sort + gather in one).
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
@@ -1625,15 +1847,16 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
   k_cells<<<g_scan, kScanThreads, 0, stream>>>(xs_a, ys_a, xs_b, ys_b, prm,
                                                at<uint64_t>(ws, L.cell_status), point_cell,
-                                               cell_start, cell_key, cell_flags, cell_box, n,
-                                               inv_r);
+                                               cell_start, cell_key, cell_flags, cell_box,
+                                               at<int2>(ws, L.col_range), L.max_cols, n, inv_r);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
   k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
                                       cell_box, cell_flags, is_core, n, ms);
 
   // Stage 4: window columns.
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
-  k_ngh<<<g_cells, 256, 0, stream>>>(prm, cell_key, ngh, cell_ref, uf);
+  k_ngh<<<g_cells, 256, 0, stream>>>(prm, cell_key, at<int2>(ws, L.col_range), L.max_cols, ngh,
+                                     cell_ref, uf);
 
   // Stage 5: core marking.
   if ((e = before(kKCore)) != hipSuccess) return fail(e);

@@ -208,3 +208,14 @@ def test_one_native_hip_runtime(dt):
     hip = {l.split()[-1] for l in maps if "libamdhip64" in l}
     assert len(hip) == 1, hip
     assert any("libdbscan_hip.so" in l for l in maps)
+
+
+def test_eight_byte_aligned_input(dt):
+    """A view starting one row into its storage is only 8-byte aligned; the
+    bounds and histogram kernels then read it one point at a time."""
+    X, eps, ms = W.cfg1()
+    big = torch.zeros((len(X) + 1, 2), dtype=torch.float32, device=DEV)
+    big[1:] = torch.from_numpy(X).to(DEV)
+    view = big[1:]
+    assert view.is_contiguous() and view.data_ptr() % 16 == 8
+    assert np.array_equal(dt.dbscan2d(view, eps, ms).cpu().numpy(), oracle.dbscan2d(X, eps, ms))
