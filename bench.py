@@ -1,6 +1,6 @@
 """Benchmark of the gfx950 dbscan2d path on one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3a]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload blobs10m]
 
 A step is one dbscan2d call on the resident N = 10,000,000-point
 constant-density workload (1,000 Gaussian blobs, std 0.5, centres in
@@ -108,21 +108,35 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg3a", choices=["cfg2", "cfg3a", "cfg3b"])
+    ap.add_argument("--workload", default="blobs10m", choices=["lattice1m", "blobs10m", "skewed10m"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: dbscan_torch runs on one GPU; --gpus {args.gpus} is not supported "
               "(the single-cloud path does not shard across GPUs)", file=sys.stderr)
         return 2
 
     import numpy as np
     import torch
 
+    if args.copy_calibration:
+        # a plain copy kernel of known size: each launch reads and writes 256 MiB
+        nbytes = 256 << 20
+        a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes,
+                                               "launches": 5}}))
+        return 0
+
     import dbscan_torch
     from tests import workloads as W
 
-    X, eps, ms = W.CONFIGS[args.workload]()
+    X, eps, ms = W.WORKLOADS[args.workload]()
     n = len(X)
     Xd = torch.from_numpy(X).cuda()
     for _ in range(args.warmup):
@@ -163,9 +177,9 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, resident in HBM before timing)",
-        "config": {"workload": {"cfg2": "cfg2 1.07M blobs+lattice eps0.5 ms10",
-                                "cfg3a": "cfg3a 10M constant-density eps0.5 ms10",
-                                "cfg3b": "cfg3b 10M skewed eps0.25 ms20"}[args.workload],
+        "config": {"workload": {"lattice1m": "lattice1m 1.07M blobs+lattice eps0.5 ms10",
+                                "blobs10m": "blobs10m 10M constant-density eps0.5 ms10",
+                                "skewed10m": "skewed10m 10M skewed eps0.25 ms20"}[args.workload],
                    "n_points": n, "eps": eps, "min_samples": ms, "num_cells": num_cells,
                    "sort_passes": npasses},
         "roofline": {"bound": "hbm", "kernel": dom,

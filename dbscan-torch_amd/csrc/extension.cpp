@@ -54,7 +54,7 @@ DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, fl
   c10::Device device = X.device();
   if (!X.is_cuda()) {
     TORCH_CHECK(c10::hip::device_count() > 0,
-                "dbscan2d: no HIP device is available. This build clusters on an AMD GPU "
+                "dbscan2d: no HIP device is available. dbscan_torch clusters on an AMD GPU "
                 "(gfx950); pass a tensor on a ROCm device or run on a machine with one.");
     device = c10::Device(c10::kCUDA, c10::hip::current_device());
   }

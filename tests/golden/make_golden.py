@@ -7,11 +7,11 @@ Run in the build container (where the reference tree exists):
     python tests/golden/make_golden.py
 
 Outputs (all data, no code):
-  cfg1_labels.npy   int32 labels of the 10,000-point workload, input order
+  blobs10k_labels.npy   int32 labels of the 10,000-point workload, input order
   cases.npz         small constructed inputs and their reference labels
                     (x_<name>, eps_<name>, ms_<name>, labels_<name>)
   golden.json       sha256 of the int32 label bytes, cluster count, noise
-                    count and cell count for cfg1, cfg2, cfg3a and cfg3b
+                    count and cell count for blobs10k, lattice1m, blobs10m and skewed10m
 """
 
 from __future__ import annotations
@@ -47,12 +47,12 @@ def main() -> None:
     if not oracle.ref_available():
         raise SystemExit("oracle/_ref/_C.so is not built: run make -C oracle ref")
     golden = {}
-    for name, fn in W.CONFIGS.items():
+    for name, fn in W.WORKLOADS.items():
         X, eps, ms = fn()
         ref = oracle.ref_dbscan2d(X, eps, ms)
         golden[name] = summary(X, eps, ms, ref)
-        if name == "cfg1":
-            np.save(os.path.join(HERE, "cfg1_labels.npy"), ref.astype(np.int32))
+        if name == "blobs10k":
+            np.save(os.path.join(HERE, "blobs10k_labels.npy"), ref.astype(np.int32))
         print(name, golden[name], flush=True)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(golden, f, indent=2, sort_keys=True)

@@ -89,16 +89,16 @@ def test_random_inputs_match_oracle(dt, seed):
     assert np.array_equal(gpu_labels(dt, X, eps, ms), oracle.dbscan2d(X, eps, ms))
 
 
-def test_cfg1_golden(dt, golden_dir, golden):
-    X, eps, ms = W.cfg1()
+def test_blobs10k_golden(dt, golden_dir, golden):
+    X, eps, ms = W.blobs10k()
     labels = gpu_labels(dt, X, eps, ms)
-    assert np.array_equal(labels, np.load(os.path.join(golden_dir, "cfg1_labels.npy")))
-    assert sha(labels) == golden["cfg1"]["sha256"]
+    assert np.array_equal(labels, np.load(os.path.join(golden_dir, "blobs10k_labels.npy")))
+    assert sha(labels) == golden["blobs10k"]["sha256"]
 
 
-@pytest.mark.parametrize("name", ["cfg2", "cfg3a", "cfg3b"])
+@pytest.mark.parametrize("name", ["lattice1m", "blobs10m", "skewed10m"])
 def test_large_configs(dt, golden, name):
-    X, eps, ms = W.CONFIGS[name]()
+    X, eps, ms = W.WORKLOADS[name]()
     labels = gpu_labels(dt, X, eps, ms)
     g = golden[name]
     assert int(labels.max()) + 1 == g["n_clusters"]
@@ -108,15 +108,15 @@ def test_large_configs(dt, golden, name):
     assert error_flag(dt, X, eps, ms) == 0
 
 
-def test_permutation_equivariance_cfg2(dt):
-    X, eps, ms = W.cfg2()
+def test_permutation_equivariance_lattice1m(dt):
+    X, eps, ms = W.lattice1m()
     base = gpu_labels(dt, X, eps, ms)
     perm = np.random.default_rng(9).permutation(len(X))
     assert np.array_equal(gpu_labels(dt, X[perm], eps, ms), base[perm])
 
 
-def test_repeated_runs_identical_cfg3b(dt):
-    X, eps, ms = W.cfg3b()
+def test_repeated_runs_identical_skewed10m(dt):
+    X, eps, ms = W.skewed10m()
     Xd = torch.from_numpy(X).to(DEV)
     first = dt.dbscan2d(Xd, eps, ms)
     for _ in range(2):
@@ -126,7 +126,7 @@ def test_repeated_runs_identical_cfg3b(dt):
 def test_stage_intermediates_match_oracle(dt):
     """Core flags, cell count, cell core status and cluster count agree with
     the oracle's stages (compared per original point index)."""
-    X, eps, ms = W.cfg2()
+    X, eps, ms = W.lattice1m()
     ref = oracle.dbscan2d_debug(X, eps, ms)
     dbg = [t.cpu().numpy() for t in dt._C.dbscan2d_hip_debug(torch.from_numpy(X).to(DEV), eps, ms)]
     labels, info, perm, xs, ys, point_cell, is_core = dbg[:7]
@@ -155,14 +155,14 @@ def test_stage_intermediates_match_oracle(dt):
 
 
 def test_host_tensor_in_host_tensor_out(dt):
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     out = dt.dbscan2d(torch.from_numpy(X), eps, ms)
     assert out.device.type == "cpu" and out.dtype == torch.int32
     assert np.array_equal(out.numpy(), oracle.dbscan2d(X, eps, ms))
 
 
 def test_profile_variant(dt):
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     labels, stages = dt._C.dbscan2d_profile(torch.from_numpy(X).to(DEV), eps, ms)
     assert np.array_equal(labels.cpu().numpy(), oracle.dbscan2d(X, eps, ms))
     assert len(stages) == 10 and all(v >= 0.0 for v in stages.values())
@@ -177,7 +177,7 @@ def test_c_abi_direct(dt):
     lib.dbscan2d_hip.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.c_int64, P, P,
                                  ctypes.c_size_t, P]
     lib.dbscan2d_hip_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     n = len(X)
     Xd = torch.from_numpy(X).to(DEV)
     labels = torch.full((n,), 7, dtype=torch.int32, device=DEV)
@@ -213,7 +213,7 @@ def test_one_native_hip_runtime(dt):
 def test_eight_byte_aligned_input(dt):
     """A view starting one row into its storage is only 8-byte aligned; the
     bounds and histogram kernels then read it one point at a time."""
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     big = torch.zeros((len(X) + 1, 2), dtype=torch.float32, device=DEV)
     big[1:] = torch.from_numpy(X).to(DEV)
     view = big[1:]

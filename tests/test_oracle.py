@@ -2,14 +2,14 @@
 against it.  CPU only.
 
 * golden vectors written by the reference's own compiled CPU path
-  (tests/golden/make_golden.py): cfg1 labels, constructed cases, label hashes;
+  (tests/golden/make_golden.py): blobs10k labels, constructed cases, label hashes;
 * the compiled reference itself (oracle/_ref), where it is built;
 * an O(N^2) brute force that states each reference rule directly
   (tests/bruteforce.py), on random and constructed inputs;
 * closed forms and special cases (empty, ms = 1, ms > N, single point,
   narrowing of min_samples to int32);
 * invariants: permutation equivariance and thread-count independence;
-* sklearn: core and noise sets on cfg1.
+* sklearn: core and noise sets on blobs10k.
 Each check is chosen so that a plausible mistake (a dropped own-cell term, a
 wrong tiebreak, unsigned key order, an off-by-one window) fails it; see
 test_planted_mistakes_are_caught.
@@ -38,18 +38,18 @@ def golden(golden_dir):
         return json.load(f)
 
 
-def test_cfg1_matches_reference_golden(golden_dir, golden):
-    X, eps, ms = W.cfg1()
+def test_blobs10k_matches_reference_golden(golden_dir, golden):
+    X, eps, ms = W.blobs10k()
     labels = oracle.dbscan2d(X, eps, ms)
-    ref = np.load(os.path.join(golden_dir, "cfg1_labels.npy"), allow_pickle=False)
+    ref = np.load(os.path.join(golden_dir, "blobs10k_labels.npy"), allow_pickle=False)
     assert np.array_equal(labels, ref)
-    assert sha(labels) == golden["cfg1"]["sha256"]
+    assert sha(labels) == golden["blobs10k"]["sha256"]
 
 
-def test_cfg2_matches_reference_hash(golden):
-    X, eps, ms = W.cfg2()
+def test_lattice1m_matches_reference_hash(golden):
+    X, eps, ms = W.lattice1m()
     labels = oracle.dbscan2d(X, eps, ms)
-    g = golden["cfg2"]
+    g = golden["lattice1m"]
     assert (int(labels.max()) + 1, int((labels == -1).sum())) == (g["n_clusters"], g["n_noise"])
     assert sha(labels) == g["sha256"]
 
@@ -135,14 +135,14 @@ def test_special_cases():
 
 
 def test_permutation_equivariance():
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     base = oracle.dbscan2d(X, eps, ms)
     perm = np.random.default_rng(5).permutation(len(X))
     assert np.array_equal(oracle.dbscan2d(X[perm], eps, ms), base[perm])
 
 
 def test_thread_count_independent():
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     before = oracle.set_threads(0)
     try:
         oracle.set_threads(1)
@@ -154,12 +154,12 @@ def test_thread_count_independent():
     assert np.array_equal(a, b)
 
 
-def test_cfg1_core_and_noise_match_sklearn():
+def test_blobs10k_core_and_noise_match_sklearn():
     """Core and noise sets equal sklearn's except where sklearn's float64
     distance and the reference's float32 rule (or its own-cell rule) disagree
     on a pair at the eps boundary."""
     sk = pytest.importorskip("sklearn.cluster")
-    X, eps, ms = W.cfg1()
+    X, eps, ms = W.blobs10k()
     dbg = oracle.dbscan2d_debug(X, eps, ms)
     core = np.zeros(len(X), bool)
     core[dbg["perm"]] = dbg["is_core"].astype(bool)

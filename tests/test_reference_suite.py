@@ -1,8 +1,8 @@
 """The reference's own test suite (tests/test_basic.py, test_eps_boundary.py,
 test_sklearn_parity.py, test_determinism.py, test_class_wrapper.py), run
-against this build on both tensor placements.  Every call computes on the GPU
+against dbscan_torch on both tensor placements.  Every call computes on the GPU
 (host tensors are staged), so the whole module is marked gpu.  Where the
-reference only asks for partition agreement with the CUDA path, this build is
+reference only asks for partition agreement with the CUDA path, the HIP path is
 held to raw equality with the oracle instead."""
 
 import numpy as np

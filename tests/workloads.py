@@ -7,17 +7,17 @@ extra point), then ``n_noise = round(noise_frac * n)`` uniform noise points in
 the noise box; it concatenates them, casts to float32 and shuffles the rows.
 
 Named workloads:
-  cfg1   10,000 pts: 8 blobs std 0.5 in [-10,10]^2 + 5% noise in [-12,12]^2,
+  blobs10k   10,000 pts: 8 blobs std 0.5 in [-10,10]^2 + 5% noise in [-12,12]^2,
          eps 0.3, min_samples 10, seed 0
-  cfg2   1,000,000 pts: 100 blobs std 1.0 in [0,400]^2 + 10% noise, seed 1,
+  lattice1m   1,000,000 pts: 100 blobs std 1.0 in [0,400]^2 + 10% noise, seed 1,
          then the 256x256 lattice {(0.5 i, 0.5 j)} appended in i-major order
          (N = 1,065,536); eps 0.5, min_samples 10
-  cfg3a  10,000,000 pts: 1,000 blobs std 0.5 in [0,1265]^2 + 5% noise in
+  blobs10m  10,000,000 pts: 1,000 blobs std 0.5 in [0,1265]^2 + 5% noise in
          [-126.5,1391.5]^2, seed 2; eps 0.5, min_samples 10
-  cfg3b  10,000,000 pts, seed 3: 16 centres then 400 centres uniform in
+  skewed10m  10,000,000 pts, seed 3: 16 centres then 400 centres uniform in
          [0,2000]^2; 6M points in the 16 blobs (std 0.05), 3M in the 400 blobs
          (std 1.0), 1M uniform in [0,2000]^2; eps 0.25, min_samples 20
-  cfg4   2,048 clouds (master seed 4): N_i = floor(exp(U(ln 200, ln 5e5))),
+  clouds2048   2,048 clouds (master seed 4): N_i = floor(exp(U(ln 200, ln 5e5))),
          cloud i = G(N_i, max(1, round(8 N_i / 1e4)), [-L/2, L/2], 0.5, 0.05,
          [-0.6 L, 0.6 L], 1000 + i) with L = 20 sqrt(N_i / 1e4);
          eps_i = f32(U[0.2, 1.0]), min_samples_i from {5, 10, 20}
@@ -51,20 +51,20 @@ def lattice(side: int = 256, spacing: float = 0.5) -> np.ndarray:
     return np.stack([i.ravel() * spacing, j.ravel() * spacing], axis=1).astype(np.float32)
 
 
-def cfg1():
+def blobs10k():
     return G(10_000, 8, (-10, 10), 0.5, 0.05, (-12, 12), 0), 0.3, 10
 
 
-def cfg2():
+def lattice1m():
     X = G(1_000_000, 100, (0, 400), 1.0, 0.10, (0, 400), 1)
     return np.concatenate([X, lattice()]), 0.5, 10
 
 
-def cfg3a():
+def blobs10m():
     return G(10_000_000, 1000, (0, 1265), 0.5, 0.05, (-126.5, 1391.5), 2), 0.5, 10
 
 
-def cfg3b():
+def skewed10m():
     rng = np.random.default_rng(3)
     c16 = rng.uniform(0, 2000, (16, 2))
     c400 = rng.uniform(0, 2000, (400, 2))
@@ -76,7 +76,7 @@ def cfg3b():
     return X, 0.25, 20
 
 
-def cfg4_params(n_clouds: int = 2048, seed: int = 4):
+def clouds2048_params(n_clouds: int = 2048, seed: int = 4):
     """Per-cloud (N_i, eps_i, min_samples_i) of the batched workload."""
     rng = np.random.default_rng(seed)
     ns = np.floor(np.exp(rng.uniform(np.log(200), np.log(500_000), n_clouds))).astype(np.int64)
@@ -85,13 +85,13 @@ def cfg4_params(n_clouds: int = 2048, seed: int = 4):
     return ns, eps, ms
 
 
-def cfg4_cloud(i: int, n_i: int) -> np.ndarray:
+def clouds2048_cloud(i: int, n_i: int) -> np.ndarray:
     L = 20.0 * np.sqrt(n_i / 1e4)
     nb = max(1, int(round(8 * n_i / 1e4)))
     return G(int(n_i), nb, (-L / 2, L / 2), 0.5, 0.05, (-0.6 * L, 0.6 * L), 1000 + i)
 
 
-CONFIGS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3a": cfg3a, "cfg3b": cfg3b}
+WORKLOADS = {"blobs10k": blobs10k, "lattice1m": lattice1m, "blobs10m": blobs10m, "skewed10m": skewed10m}
 
 
 def cell_keys_np(X: np.ndarray, eps: float):
