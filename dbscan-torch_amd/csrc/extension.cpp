@@ -158,24 +158,72 @@ std::vector<torch::Tensor> dbscan2d_hip_debug(torch::Tensor X, double eps, int64
   return out;
 }
 
-// Benchmark helper (no reference counterpart): `steps` back-to-back calls on
-// one resident input, with HIP events on the launch stream before every
-// kernel.  Returns (labels, per-kernel mean ms over the steps, kernel names,
-// event-measured ms from the first kernel of step 0 to the end of the last).
-std::tuple<torch::Tensor, std::vector<double>, std::vector<std::string>, double>
-dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t steps) {
-  check_input(X, eps, min_samples);
-  TORCH_CHECK(X.is_cuda() && X.size(0) > 0 && steps >= 1,
-              "dbscan2d_benchmark: needs a non-empty ROCm tensor and steps >= 1");
-  c10::DeviceGuard guard(X.device());
-  auto Xd = X.contiguous();
-  const int64_t n = Xd.size(0);
+namespace {
+
+// Checks a batch and returns its host arrays (no reference counterpart; the
+// per-cloud checks and messages are the single call's, naming the cloud).
+struct Batch {
+  std::vector<int64_t> offsets, ms;
+  std::vector<double> eps;
+};
+
+Batch check_batch(const torch::Tensor& X, const std::vector<int64_t>& offsets,
+                  const std::vector<double>& eps, const std::vector<int64_t>& min_samples) {
+  TORCH_CHECK(X.dim() == 2 && X.size(1) == 2, "X must have shape (N, 2)");
+  TORCH_CHECK(X.scalar_type() == torch::kFloat32, "X must be float32");
+  const size_t B = eps.size();
+  TORCH_CHECK(min_samples.size() == B, "dbscan2d_batched: eps and min_samples differ in length");
+  TORCH_CHECK(offsets.size() == B + 1, "dbscan2d_batched: offsets must hold one more value than eps");
+  TORCH_CHECK(offsets.front() == 0 && offsets.back() == X.size(0),
+              "dbscan2d_batched: offsets must run from 0 to N");
+  for (size_t i = 0; i < B; ++i) {
+    TORCH_CHECK(offsets[i + 1] >= offsets[i], "dbscan2d_batched: offsets must be non-decreasing");
+    TORCH_CHECK(eps[i] > 0.0, "eps must be > 0 (cloud ", i, ")");
+    TORCH_CHECK(min_samples[i] >= 1, "min_samples must be >= 1 (cloud ", i, ")");
+  }
+  TORCH_CHECK(X.size(0) < INT32_MAX, "dbscan2d_batched: fewer than 2^31 - 1 points in all");
+  return Batch{offsets, min_samples, eps};
+}
+
+}  // namespace
+
+// Batched clustering (no reference counterpart): every cloud i = rows
+// [offsets[i], offsets[i+1]) of X is clustered with its own eps[i] and
+// min_samples[i] in one pipeline; the labels of each cloud equal those of a
+// dbscan2d call on that cloud alone.
+torch::Tensor dbscan2d_batched(torch::Tensor X, std::vector<int64_t> offsets,
+                               std::vector<double> eps, std::vector<int64_t> min_samples) {
+  const Batch b = check_batch(X, offsets, eps, min_samples);
+  const int64_t n = X.size(0), B = static_cast<int64_t>(eps.size());
+  if (n == 0) return torch::empty({0}, X.options().dtype(torch::kInt32));
+  c10::Device device = X.device();
+  if (!X.is_cuda()) {
+    TORCH_CHECK(c10::hip::device_count() > 0,
+                "dbscan2d: no HIP device is available. dbscan_torch clusters on an AMD GPU "
+                "(gfx950); pass a tensor on a ROCm device or run on a machine with one.");
+    device = c10::Device(c10::kCUDA, c10::hip::current_device());
+  }
+  c10::DeviceGuard guard(device);
+  auto Xd = X.to(device).contiguous();
   size_t ws_bytes = 0;
-  check_status(dbscan2d_hip_workspace_bytes(n, &ws_bytes));
-  auto opts = torch::TensorOptions().device(X.device());
+  check_status(dbscan2d_hip_batched_workspace_bytes(n, B, &ws_bytes));
+  auto opts = torch::TensorOptions().device(device);
   auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
   auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
-  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  hipStream_t stream = c10::hip::getCurrentHIPStream(device.index()).stream();
+  int bad = -1;
+  check_status(dbscan2d_hip_batched(Xd.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
+                                    b.ms.data(), B, labels.data_ptr<int32_t>(),
+                                    workspace.data_ptr(), ws_bytes, stream, &bad));
+  return finish(X, labels);
+}
+
+namespace {
+
+// `steps` timed calls of `launch(events)`; returns per-kernel means and the
+// total event time.
+template <class F>
+std::pair<std::vector<double>, double> time_kernels(int64_t steps, F launch) {
   constexpr int K = DBSCAN_HIP_NUM_KERNELS + 1;
   std::vector<hipEvent_t> ev(static_cast<size_t>(steps) * K, nullptr);
   auto destroy = [&]() {
@@ -189,9 +237,7 @@ dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t ste
     }
   }
   for (int64_t s = 0; s < steps; ++s) {
-    const int rc = dbscan2d_hip_timed(Xd.data_ptr<float>(), n, eps, min_samples,
-                                      labels.data_ptr<int32_t>(), workspace.data_ptr(), ws_bytes,
-                                      stream, ev.data() + s * K);
+    const int rc = launch(ev.data() + s * K);
     if (rc != DBSCAN_HIP_OK) destroy();
     check_status(rc);
   }
@@ -210,9 +256,66 @@ dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t ste
   float total = 0.f;
   (void)hipEventElapsedTime(&total, ev.front(), ev.back());
   destroy();
+  return {mean, static_cast<double>(total)};
+}
+
+std::vector<std::string> kernel_names() {
   std::vector<std::string> names;
   for (int k = 0; k < DBSCAN_HIP_NUM_KERNELS; ++k) names.emplace_back(dbscan2d_hip_kernel_name(k));
-  return {labels, mean, names, static_cast<double>(total)};
+  return names;
+}
+
+}  // namespace
+
+// Benchmark helper (no reference counterpart): `steps` back-to-back calls on
+// one resident input, with HIP events on the launch stream before every
+// kernel.  Returns (labels, per-kernel mean ms over the steps, kernel names,
+// event-measured ms from the first kernel of step 0 to the end of the last).
+std::tuple<torch::Tensor, std::vector<double>, std::vector<std::string>, double>
+dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t steps) {
+  check_input(X, eps, min_samples);
+  TORCH_CHECK(X.is_cuda() && X.size(0) > 0 && steps >= 1,
+              "dbscan2d_benchmark: needs a non-empty ROCm tensor and steps >= 1");
+  c10::DeviceGuard guard(X.device());
+  auto Xd = X.contiguous();
+  const int64_t n = Xd.size(0);
+  size_t ws_bytes = 0;
+  check_status(dbscan2d_hip_workspace_bytes(n, &ws_bytes));
+  auto opts = torch::TensorOptions().device(X.device());
+  auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
+  auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
+  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  auto r = time_kernels(steps, [&](hipEvent_t* ev) {
+    return dbscan2d_hip_timed(Xd.data_ptr<float>(), n, eps, min_samples,
+                              labels.data_ptr<int32_t>(), workspace.data_ptr(), ws_bytes, stream,
+                              ev);
+  });
+  return {labels, r.first, kernel_names(), r.second};
+}
+
+// Benchmark helper for batches (no reference counterpart), as
+// dbscan2d_benchmark: `steps` back-to-back batched calls on resident input.
+std::tuple<torch::Tensor, std::vector<double>, std::vector<std::string>, double>
+dbscan2d_batched_benchmark(torch::Tensor X, std::vector<int64_t> offsets, std::vector<double> eps,
+                           std::vector<int64_t> min_samples, int64_t steps) {
+  const Batch b = check_batch(X, offsets, eps, min_samples);
+  TORCH_CHECK(X.is_cuda() && X.size(0) > 0 && steps >= 1,
+              "dbscan2d_batched_benchmark: needs a non-empty ROCm tensor and steps >= 1");
+  c10::DeviceGuard guard(X.device());
+  auto Xd = X.contiguous();
+  const int64_t n = Xd.size(0), B = static_cast<int64_t>(eps.size());
+  size_t ws_bytes = 0;
+  check_status(dbscan2d_hip_batched_workspace_bytes(n, B, &ws_bytes));
+  auto opts = torch::TensorOptions().device(X.device());
+  auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
+  auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
+  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  auto r = time_kernels(steps, [&](hipEvent_t* ev) {
+    return dbscan2d_hip_batched_timed(Xd.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
+                                      b.ms.data(), B, labels.data_ptr<int32_t>(),
+                                      workspace.data_ptr(), ws_bytes, stream, ev);
+  });
+  return {labels, r.first, kernel_names(), r.second};
 }
 
 // csrc/extension.cpp:65-81
@@ -233,4 +336,13 @@ PYBIND11_MODULE(_C, m) {
         pybind11::arg("min_samples"), pybind11::arg("steps"),
         "Benchmark helper: (labels, per-kernel mean ms, kernel names, total event ms) over "
         "`steps` back-to-back calls. Not part of the public API.");
+  m.def("dbscan2d_batched", &dbscan2d_batched, pybind11::arg("X"), pybind11::arg("offsets"),
+        pybind11::arg("eps"), pybind11::arg("min_samples"),
+        "Batched DBSCAN: cloud i = rows [offsets[i], offsets[i+1]) of the packed (N, 2) float32 "
+        "X, clustered with eps[i], min_samples[i]. Returns the packed int32 labels, numbered "
+        "from 0 within every cloud, on X's device.");
+  m.def("dbscan2d_batched_benchmark", &dbscan2d_batched_benchmark, pybind11::arg("X"),
+        pybind11::arg("offsets"), pybind11::arg("eps"), pybind11::arg("min_samples"),
+        pybind11::arg("steps"),
+        "Benchmark helper for batches, as dbscan2d_benchmark. Not part of the public API.");
 }

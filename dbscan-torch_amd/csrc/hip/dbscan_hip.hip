@@ -42,6 +42,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "dbscan_hip.h"
 
@@ -50,7 +51,9 @@ namespace {
 constexpr int32_t kNoise = -1;
 // Radix digits are at most kRadixBitsMax wide.  A key of b bits is sorted in
 // P = ceil(b / kRadixBitsMax) passes of ceil(b / P) bits each, both chosen on
-// the device once the bounds are known (26-bit keys: 3 passes of 9, 9, 8).
+// the device once the bounds are known (26-bit keys: 4 passes of 7 bits).  In
+// a batch every cloud takes the batch's largest P and splits its own key bits
+// over those P passes.
 constexpr int kRadixBitsMax = 8;
 constexpr int kRadixMax = 1 << kRadixBitsMax;
 constexpr int kMaxPasses = 8;           // 64-bit keys at most
@@ -90,7 +93,7 @@ struct Params {
   int32_t num_cells;
   int32_t npasses;
   int32_t error;
-  int32_t pad0;
+  int32_t col_total;  // batches: column-index entries in use
   // Everything below is cleared by k_bounds at the start of every call.
   uint32_t tile_counter[16];
   uint32_t hist[kMaxPasses][kRadixMax];
@@ -119,15 +122,16 @@ __device__ __forceinline__ uint32_t bit_width(uint32_t v) {
   return v ? 32u - static_cast<uint32_t>(__clz(static_cast<int>(v))) : 0u;
 }
 
+// passes > 0: split the key over that many passes (a batch's common count).
 __device__ __forceinline__ KeyMap make_keymap(int32_t kxmin, int32_t kxmax, int32_t kymin,
-                                              int32_t kymax) {
+                                              int32_t kymax, int passes = 0) {
   KeyMap m;
   m.kxmin = static_cast<uint32_t>(kxmin);
   m.kymin = static_cast<uint32_t>(kymin);
   m.bx = bit_width(static_cast<uint32_t>(kxmax) - m.kxmin);
   m.by = bit_width(static_cast<uint32_t>(kymax) - m.kymin);
   const int nbits = static_cast<int>(m.bx + m.by);
-  m.npasses = nbits ? (nbits + kRadixBitsMax - 1) / kRadixBitsMax : 1;
+  m.npasses = passes > 0 ? passes : (nbits ? (nbits + kRadixBitsMax - 1) / kRadixBitsMax : 1);
   m.dbits = nbits ? static_cast<uint32_t>((nbits + m.npasses - 1) / m.npasses) : 1u;
   return m;
 }
@@ -135,6 +139,52 @@ __device__ __forceinline__ KeyMap make_keymap(int32_t kxmin, int32_t kxmax, int3
 __device__ __forceinline__ KeyMap load_keymap(const Params* p) {
   return make_keymap(p->kxmin, p->kxmax, p->kymin, p->kymax);
 }
+
+// One cloud of a batch (dbscan2d_hip_batched): its point range in the packed
+// input, its scalar parameters (computed on the host exactly as for a single
+// call), and what the kernels derive for it.  A single call is not a batch
+// and keeps its parameters in kernel arguments and Params.
+struct CloudInfo {
+  int32_t begin, end;            // points [begin, end) of the packed input
+  float inv_r, eps_sq, best_init;
+  int32_t ms;                    // narrowed min_samples; INT32_MAX: every point is noise
+  int32_t kxmin, kxmax, kymin, kymax;  // cell bounds (k_bounds_seg, atomics)
+  int32_t cell_begin, cell_end;  // its cells (k_cells)
+  int32_t col_base;              // first column-index entry, or -1 (binary search)
+  int32_t pad[3];
+};
+static_assert(sizeof(CloudInfo) == 64, "CloudInfo is 64 bytes");
+
+// A tile of consecutive points of one cloud.  `first` is the global index of
+// the cloud's first tile (look-backs stop there).
+struct SegTile {
+  int32_t cloud, start, count, first;
+};
+
+__device__ __forceinline__ KeyMap cloud_keymap(const CloudInfo& ci, int passes) {
+  return make_keymap(ci.kxmin, ci.kxmax, ci.kymin, ci.kymax, passes);
+}
+
+// Scalar parameters of the cloud a cell belongs to.  The per-cell kernels
+// take them from one of two sources: OneCloud (a single call: kernel
+// arguments) or ManyClouds (a batch: the cell's cloud record).
+struct CellParams {
+  float eps_sq, best_init;
+  int32_t ms;
+};
+struct OneCloud {
+  float eps_sq, best_init;
+  int32_t ms;
+  __device__ __forceinline__ CellParams at(int32_t) const { return {eps_sq, best_init, ms}; }
+};
+struct ManyClouds {
+  const int32_t* __restrict__ cell_cloud;
+  const CloudInfo* __restrict__ clouds;
+  __device__ __forceinline__ CellParams at(int32_t c) const {
+    const CloudInfo& ci = clouds[cell_cloud[c]];
+    return {ci.eps_sq, ci.best_init, ci.ms};
+  }
+};
 
 // Column index: for x' < 2^bx <= max_cols, col_range[x'] = (first cell, end
 // cell) of column x' in cell order (both 0 for an empty column).
@@ -368,6 +418,7 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
     prm->npasses = m.npasses;
     prm->num_cells = 0;
     prm->error = 0;
+    prm->col_total = 0;
   }
   for (int e = threadIdx.x; e < kMaxPasses * kRadixMax; e += 256) (&s_hist[0][0])[e] = 0u;
   __syncthreads();
@@ -403,23 +454,157 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
   }
 }
 
+// ---- Batches: per-cloud bounds, key maps and histograms -------------------
+
+// Cell-coordinate bounds of each cloud: one block per tile, merged into the
+// cloud's record with atomics (the record holds INT_MAX / INT_MIN on entry).
+// Also clears the words the later kernels accumulate into.
+__global__ __launch_bounds__(256) void k_bounds_seg(const float2* __restrict__ X,
+                                                    const SegTile* __restrict__ tiles,
+                                                    CloudInfo* __restrict__ clouds,
+                                                    uint4* __restrict__ zero, int64_t zero16) {
+  const int64_t gtid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
+  for (int64_t z = gtid; z < zero16; z += stride) zero[z] = make_uint4(0, 0, 0, 0);
+  const SegTile st = tiles[blockIdx.x];
+  const float inv_r = clouds[st.cloud].inv_r;
+  int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  for (int i = threadIdx.x; i < st.count; i += 256) {
+    const float2 p = X[st.start + i];
+    const int32_t kx = cell_coord(p.x * inv_r), ky = cell_coord(p.y * inv_r);
+    v[0] = min(v[0], kx); v[1] = max(v[1], kx);
+    v[2] = min(v[2], ky); v[3] = max(v[3], ky);
+  }
+  __shared__ int32_t s[4][4];
+  block_minmax4(v, s);
+  if (threadIdx.x == 0) {
+    CloudInfo* ci = clouds + st.cloud;
+    atomicMin(&ci->kxmin, v[0]);
+    atomicMax(&ci->kxmax, v[1]);
+    atomicMin(&ci->kymin, v[2]);
+    atomicMax(&ci->kymax, v[3]);
+  }
+}
+
+// One block: the batch's pass count (the largest any cloud needs) and each
+// cloud's column-index base (an exclusive scan of 2^bx over the clouds; a
+// cloud whose columns would pass max_cols searches instead).
+__global__ __launch_bounds__(1024) void k_keys_seg(Params* __restrict__ prm,
+                                                   CloudInfo* __restrict__ clouds,
+                                                   int32_t num_clouds, int64_t max_cols) {
+  __shared__ int32_t s_max[16];
+  __shared__ uint32_t s_scan[16];
+  __shared__ int64_t s_run;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int32_t pmax = 1;
+  for (int32_t i = t; i < num_clouds; i += 1024) {
+    const CloudInfo& ci = clouds[i];
+    if (ci.end > ci.begin) pmax = max(pmax, cloud_keymap(ci, 0).npasses);
+  }
+  for (int d = 32; d >= 1; d >>= 1) pmax = max(pmax, __shfl_xor(pmax, d, 64));
+  if (lane == 0) s_max[w] = pmax;
+  if (t == 0) s_run = 0;
+  __syncthreads();
+  if (t == 0) {
+    int32_t m = 1;
+    for (int i = 0; i < 16; ++i) m = max(m, s_max[i]);
+    prm->npasses = m;
+    prm->num_cells = 0;
+    prm->error = 0;
+  }
+  for (int32_t b0 = 0; b0 < num_clouds; b0 += 1024) {
+    const int32_t i = b0 + t;
+    uint32_t cols = 0;
+    if (i < num_clouds) {
+      const CloudInfo& ci = clouds[i];
+      const uint32_t bx = cloud_keymap(ci, 1).bx;
+      // a cloud of more than 2^20 columns always searches
+      if (ci.end > ci.begin && bx <= 20) cols = 1u << bx;
+    }
+    uint32_t tot;
+    const uint32_t off = block_excl_scan<1024>(cols, tot, s_scan);
+    const int64_t at = s_run + off;
+    if (i < num_clouds)
+      clouds[i].col_base = (cols && at + cols <= max_cols) ? static_cast<int32_t>(at) : -1;
+    __syncthreads();
+    if (t == 0) s_run += tot;
+    __syncthreads();
+  }
+  if (t == 0) prm->col_total = static_cast<int32_t>(min<int64_t>(s_run, max_cols));
+}
+
+// Digit histograms of every pass, per cloud.  Block b takes a contiguous run
+// of tiles and accumulates in LDS while the cloud stays the same, flushing the
+// non-zero counts to the cloud's histogram when it changes.  Also clears the
+// column index.
+__global__ __launch_bounds__(256) void k_hist_seg(const float2* __restrict__ X,
+                                                  const SegTile* __restrict__ tiles,
+                                                  int32_t num_tiles, int32_t tiles_per_block,
+                                                  const CloudInfo* __restrict__ clouds,
+                                                  const Params* __restrict__ prm,
+                                                  uint32_t* __restrict__ cloud_hist,
+                                                  int2* __restrict__ col_range) {
+  __shared__ uint32_t s_hist[kMaxPasses][kRadixMax];
+  const int t = threadIdx.x;
+  const int P = prm->npasses;
+  const int64_t gtid = static_cast<int64_t>(blockIdx.x) * 256 + t;
+  for (int64_t q = gtid; q < prm->col_total; q += static_cast<int64_t>(gridDim.x) * 256)
+    col_range[q] = make_int2(0, 0);
+  for (int e = t; e < kMaxPasses * kRadixMax; e += 256) (&s_hist[0][0])[e] = 0u;
+  const int32_t t0 = blockIdx.x * tiles_per_block;
+  const int32_t t1 = min(num_tiles, t0 + tiles_per_block);
+  int32_t cur = -1;
+  auto flush = [&]() {
+    __syncthreads();
+    if (cur >= 0) {
+      uint32_t* h = cloud_hist + static_cast<size_t>(cur) * kMaxPasses * kRadixMax;
+      for (int e = t; e < P * kRadixMax; e += 256) {
+        const uint32_t c = (&s_hist[0][0])[e];
+        if (c) {
+          atomicAdd(h + e, c);
+          (&s_hist[0][0])[e] = 0u;
+        }
+      }
+    }
+    __syncthreads();
+  };
+  for (int32_t ti = t0; ti < t1; ++ti) {
+    const SegTile st = tiles[ti];
+    if (st.cloud != cur) {
+      flush();
+      cur = st.cloud;
+    }
+    const CloudInfo& ci = clouds[st.cloud];
+    const KeyMap m = cloud_keymap(ci, P);
+    for (int i = t; i < st.count; i += 256) {
+      const float2 p = X[st.start + i];
+      const uint64_t key = sort_key(p.x, p.y, ci.inv_r, m);
+      for (int ps = 0; ps < P; ++ps) atomicAdd(&s_hist[ps][key_digit(key, ps, m)], 1u);
+    }
+  }
+  flush();
+}
+
 // Sum of the earlier tiles' counts of one digit (decoupled look-back).  The
 // status words of four predecessors are read per round trip; a word tagged
 // `tag_pre` is an inclusive prefix (stop), `tag_agg` a tile's own count
 // (add, keep going), anything else is not yet published (or left by an
 // earlier pass) and is waited for, within kSpinLimit.
+// Tiles below `first` (another cloud of a batch) are not part of the sum.
 __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ status,
-                                                   int64_t tile, int digit, uint64_t tag_agg,
-                                                   uint64_t tag_pre, int32_t* error) {
+                                                   int64_t tile, int64_t first, int digit,
+                                                   uint64_t tag_agg, uint64_t tag_pre,
+                                                   int32_t* error) {
   uint64_t excl = 0;
   int64_t j = tile - 1;
   uint32_t spins = 0;
-  while (j >= 0) {
+  while (j >= first) {
     uint64_t w[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      w[k] = (j - k >= 0) ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadixMax + digit)
-                          : tag_pre;
+      w[k] = (j - k >= first)
+                 ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadixMax + digit)
+                 : tag_pre;
     // consume the published words in order: `used` of them, `stop` at a prefix
     int used = 0;
     bool stop = false, blocked = false;
@@ -458,14 +643,38 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 // then reordered by digit through one LDS buffer, array by array, so that runs
 // of equal digits leave as contiguous writes.  Thread t owns digits
 // [t*kDigitsPerThread, (t+1)*kDigitsPerThread).
-template <bool kFirst>
+// kSeg (batches): tile `blockIdx.x` is tiles[blockIdx.x], a run of one
+// cloud's points; the cloud's own histogram, key map and scale apply, the
+// look-back stops at the cloud's first tile, and the cloud's points stay in
+// its own range [begin, end), so every cloud is sorted on its own.
+template <bool kFirst, bool kSeg>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
-    int pass, int64_t n64, float inv_r) {
+    int pass, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
+    const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist) {
   const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31: 32-bit offsets
-  const KeyMap m = load_keymap(prm);
+  const uint32_t tile = blockIdx.x;  // see kTileOrder
+  KeyMap m;
+  uint32_t base, tile_n, out_base = 0, first = 0;
+  const uint32_t* __restrict__ hist;
+  if constexpr (kSeg) {
+    const SegTile st = tiles[tile];
+    const CloudInfo& ci = clouds[st.cloud];
+    m = cloud_keymap(ci, prm->npasses);
+    base = static_cast<uint32_t>(st.start);
+    tile_n = static_cast<uint32_t>(st.count);
+    out_base = static_cast<uint32_t>(ci.begin);
+    first = static_cast<uint32_t>(st.first);
+    inv_r = ci.inv_r;
+    hist = cloud_hist + (static_cast<size_t>(st.cloud) * kMaxPasses + pass) * kRadixMax;
+  } else {
+    m = load_keymap(prm);
+    base = tile * kSortTile;
+    tile_n = min(static_cast<uint32_t>(kSortTile), n - base);
+    hist = prm->hist[pass];
+  }
   if (pass >= m.npasses) return;
   constexpr int D = kDigitsPerThread;
 
@@ -484,20 +693,19 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     uint32_t h[D], sum = 0;
 #pragma unroll
     for (int q = 0; q < D; ++q) {
-      h[q] = prm->hist[pass][t * D + q];
+      h[q] = hist[t * D + q];
       sum += h[q];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan<kSortThreads>(sum, tot, s_scan);
+    uint32_t run = out_base + block_excl_scan<kSortThreads>(sum, tot, s_scan);
 #pragma unroll
     for (int q = 0; q < D; ++q) {
       goff[q] = run;
       run += h[q];
     }
   }
-  const uint32_t tile = blockIdx.x;  // see kTileOrder
-  const uint32_t base = tile * kSortTile;
-  const uint32_t wbase = base + static_cast<uint32_t>(wave) * (64 * kSortItems);
+  // local index of item k of this lane: wl + k * 64
+  const uint32_t wl = static_cast<uint32_t>(wave) * (64 * kSortItems) + lane;
 
   float x[kSortItems], y[kSortItems];
   uint32_t id[kSortItems], dg[kSortItems], rk[kSortItems];
@@ -505,8 +713,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   for (int k = 0; k < kSortItems; ++k) {
     // unconditional loads (a clamped index past the end) keep the per-item
     // values in plain registers; validity is tested where it matters
-    const uint32_t e = wbase + k * 64 + lane;
-    const uint32_t ec = e < n ? e : n - 1;
+    const uint32_t l = wl + k * 64;
+    const uint32_t e = base + l;
+    const uint32_t ec = l < tile_n ? e : base + tile_n - 1;
     if (kFirst) {
       const float2 p = X[ec];
       x[k] = p.x; y[k] = p.y; id[k] = e;
@@ -525,7 +734,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    const bool valid = wbase + k * 64 + lane < n;
+    const bool valid = wl + k * 64 < tile_n;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kRadixBitsMax; ++b) {
@@ -557,7 +766,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     s_wcnt[2][d] = c0 + c1;
     s_wcnt[3][d] = c0 + c1 + c2;
     st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
-               (tile == 0 ? tag_pre : tag_agg) | dtot[q]);
+               (tile == first ? tag_pre : tag_agg) | dtot[q]);
     dsum += dtot[q];
   }
   uint32_t ttot;
@@ -566,8 +775,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   for (int q = 0; q < D; ++q) {
     const int d = t * D + q;
     uint64_t excl = 0;
-    if (tile > 0) {
-      excl = digit_lookback(status, static_cast<int64_t>(tile), d, tag_agg, tag_pre, &prm->error);
+    if (tile > first) {
+      excl = digit_lookback(status, static_cast<int64_t>(tile), static_cast<int64_t>(first), d,
+                            tag_agg, tag_pre, &prm->error);
       st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
     }
     s_dstart[d] = dstart;
@@ -581,22 +791,21 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
     rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
-    if (wbase + k * 64 + lane < n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
+    if (wl + k * 64 < tile_n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
   }
-  const int tile_n = static_cast<int>(min(static_cast<uint32_t>(kSortTile), n - base));
   // x, y and index in turn through the one buffer
 #pragma unroll
   for (int arr = 0; arr < 3; ++arr) {
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-      if (wbase + k * 64 + lane < n)
+      if (wl + k * 64 < tile_n)
         s_buf[rk[k]] = arr == 0 ? __float_as_uint(x[k]) : (arr == 1 ? __float_as_uint(y[k]) : id[k]);
     }
     __syncthreads();
     uint32_t* __restrict__ out = arr == 0 ? reinterpret_cast<uint32_t*>(out_x)
                                           : (arr == 1 ? reinterpret_cast<uint32_t*>(out_y) : out_i);
 #pragma unroll 2
-    for (int j = t; j < tile_n; j += kSortThreads) {
+    for (int j = t; j < static_cast<int>(tile_n); j += kSortThreads) {
       const uint32_t pos = s_goff[s_d[j]] + static_cast<uint32_t>(j);
       if (pos < n) out[pos] = s_buf[j];
       else if (arr == 0) atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
@@ -608,31 +817,60 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 // Cell runs over the sorted points (dbscan_grid.hpp:206-227): flags where the
 // key changes, block scan plus look-back, then point->cell, cell starts and
 // cell keys.  Blocked layout: thread t owns kScanItems consecutive points,
-// read and written with 16-byte vector accesses.
+// read and written with 16-byte vector accesses where aligned.
+// kSeg (batches): tile `blockIdx.x` is tiles[blockIdx.x], a run of one
+// cloud's points; a cloud's first point always opens a cell, keys are the
+// cloud's own, and the cell numbering runs on across clouds.  Each cloud's
+// cell range, and each cell's cloud, are recorded.
+template <bool kSeg>
 __global__ __launch_bounds__(kScanThreads) void k_cells(
     const float* __restrict__ xs_a, const float* __restrict__ ys_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b, Params* __restrict__ prm,
     uint64_t* __restrict__ status, int32_t* __restrict__ point_cell,
     int32_t* __restrict__ cell_start, uint64_t* __restrict__ cell_key,
     uint32_t* __restrict__ cell_flags, uint4* __restrict__ cell_box, int2* __restrict__ col_range,
-    int64_t max_cols, int64_t n64, float inv_r) {
+    int64_t max_cols, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
+    CloudInfo* __restrict__ clouds, int32_t* __restrict__ cell_cloud) {
   static_assert(kScanItems == 8, "two float4 per thread");
   __shared__ uint64_t s_last[kScanThreads];
   __shared__ uint32_t s_scan[kScanThreads / 64];
   __shared__ uint64_t s_prefix;
   const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31
-  const KeyMap m = load_keymap(prm);
-  const bool in_a = (m.npasses & 1) != 0;
+  const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const int t = threadIdx.x;
   const uint32_t tile = blockIdx.x;  // see kTileOrder
-  const uint32_t base = tile * kScanTile;
+  KeyMap m;
+  uint32_t base, end, cbeg, cend;  // tile [base, end) of the cloud's [cbeg, cend)
+  bool colidx;
+  int64_t col_base = 0;
+  int32_t cloud = 0;
+  if constexpr (kSeg) {
+    const SegTile st = tiles[tile];
+    cloud = st.cloud;
+    const CloudInfo& ci = clouds[cloud];
+    m = cloud_keymap(ci, prm->npasses);
+    inv_r = ci.inv_r;
+    base = static_cast<uint32_t>(st.start);
+    end = base + static_cast<uint32_t>(st.count);
+    cbeg = static_cast<uint32_t>(ci.begin);
+    cend = static_cast<uint32_t>(ci.end);
+    colidx = ci.col_base >= 0;
+    col_base = ci.col_base;
+  } else {
+    m = load_keymap(prm);
+    base = tile * kScanTile;
+    end = min(base + kScanTile, n);
+    cbeg = 0;
+    cend = n;
+    colidx = has_col_index(m, max_cols);
+  }
   const uint32_t e0 = base + static_cast<uint32_t>(t) * kScanItems;
-  const bool whole = e0 + kScanItems <= n;
+  const bool whole = e0 + kScanItems <= end;
 
   float px[kScanItems], py[kScanItems];
-  if (whole) {
+  if (whole && (e0 & 3u) == 0) {
     const float4 x0 = *reinterpret_cast<const float4*>(xs + e0);
     const float4 x1 = *reinterpret_cast<const float4*>(xs + e0 + 4);
     const float4 y0 = *reinterpret_cast<const float4*>(ys + e0);
@@ -644,7 +882,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
   } else {
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-      const uint32_t ec = e0 + j < n ? e0 + j : n - 1;  // clamped: defined, never used
+      const uint32_t ec = e0 + j < end ? e0 + j : end - 1;  // clamped: defined, never used
       px[j] = xs[ec];
       py[j] = ys[ec];
     }
@@ -659,29 +897,29 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
   s_last[t] = key[kScanItems - 1];
   __syncthreads();
   uint64_t prev = 0;
-  bool have_prev = false;
+  bool have_prev = false;  // the previous point exists and is of the same cloud
   if (t > 0) {
     prev = s_last[t - 1];
     have_prev = true;
-  } else if (base > 0) {
+  } else if (base > cbeg) {
     uint32_t xr, yr;
     rel_cell(xs[base - 1], ys[base - 1], inv_r, m, xr, yr);
     prev = cell_key64(xr, yr);
     have_prev = true;
   }
-  uint32_t flags = 0, newcol = 0, cnt = 0;
+  uint32_t flags = 0, newcol = 0, hadprev = 0, cnt = 0;
   uint32_t prev_x[kScanItems];
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    const bool f = e0 + j < n && (!have_prev || key[j] != prev);
+    const bool f = e0 + j < end && (!have_prev || key[j] != prev);
     flags |= static_cast<uint32_t>(f) << j;
     newcol |= static_cast<uint32_t>(f && (!have_prev || (key[j] >> 32) != (prev >> 32))) << j;
+    hadprev |= static_cast<uint32_t>(have_prev) << j;
     prev_x[j] = have_prev ? static_cast<uint32_t>(prev >> 32) : 0u;
     cnt += f;
     prev = key[j];
     have_prev = true;
   }
-  const bool colidx = has_col_index(m, max_cols);
   uint32_t ttot;
   const uint32_t texcl = block_excl_scan<kScanThreads>(cnt, ttot, s_scan);
   if (t == 0) s_prefix = lookback_single(status, tile, ttot, &prm->error);
@@ -697,9 +935,13 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
         cell_key[c] = key[j];
         cell_flags[c] = 0u;
         cell_box[c] = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);  // empty box for k_bbox
+        if constexpr (kSeg) {
+          cell_cloud[c] = cloud;
+          if (e0 + j == cbeg) clouds[cloud].cell_begin = c;
+        }
         if (colidx && ((newcol >> j) & 1u)) {
-          col_range[key[j] >> 32].x = c;
-          if (c > 0) col_range[prev_x[j]].y = c;
+          col_range[col_base + static_cast<int64_t>(key[j] >> 32)].x = c;
+          if ((hadprev >> j) & 1u) col_range[col_base + prev_x[j]].y = c;
         }
       } else {
         atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
@@ -707,20 +949,24 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     }
     pc[j] = c;
   }
-  if (whole) {
+  if (whole && (e0 & 3u) == 0) {
     *reinterpret_cast<int4*>(point_cell + e0) = make_int4(pc[0], pc[1], pc[2], pc[3]);
     *reinterpret_cast<int4*>(point_cell + e0 + 4) = make_int4(pc[4], pc[5], pc[6], pc[7]);
   } else {
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j)
-      if (e0 + j < n) point_cell[e0 + j] = pc[j];
+      if (e0 + j < end) point_cell[e0 + j] = pc[j];
   }
-  if (e0 <= n - 1 && n - 1 < e0 + kScanItems) {  // this thread holds the last point
-    const int j = static_cast<int>((n - 1) - e0);
+  if (e0 <= cend - 1 && cend - 1 < e0 + kScanItems && cend - 1 < end) {
+    // this thread holds the cloud's last point
+    const int j = static_cast<int>((cend - 1) - e0);
     const int32_t last = pc[j];
-    prm->num_cells = last + 1;
-    cell_start[last + 1] = static_cast<int32_t>(n);
-    if (colidx) col_range[key[j] >> 32].y = last + 1;
+    if constexpr (kSeg) clouds[cloud].cell_end = last + 1;
+    if (colidx) col_range[col_base + static_cast<int64_t>(key[j] >> 32)].y = last + 1;
+    if (cend == n) {
+      prm->num_cells = last + 1;
+      cell_start[last + 1] = static_cast<int32_t>(n);
+    }
   }
 }
 
@@ -740,38 +986,63 @@ __device__ __forceinline__ int32_t lower_bound_u64(const uint64_t* __restrict__ 
 // the 25-cell window of the reference (dbscan_grid.hpp:239-252) is 5 runs,
 // and their order (dx-major, then ky ascending) is the reference's window
 // offset order off = (dx+2)*5 + (dy+2).  With a column index the searches
-// run inside one column; otherwise over all cells.
+// run inside one column; otherwise over all of the cloud's cells.
+// kSeg (batches): a cell's cloud supplies its cell range, bounds and column
+// index; reference indices stay inside the cloud's cell range.
+template <bool kSeg>
 __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
                                              const uint64_t* __restrict__ cell_key,
                                              const int2* __restrict__ col_range, int64_t max_cols,
                                              int32_t* __restrict__ ngh,
                                              int32_t* __restrict__ cell_ref,
-                                             int32_t* __restrict__ uf) {
+                                             int32_t* __restrict__ uf,
+                                             const int32_t* __restrict__ cell_cloud,
+                                             const CloudInfo* __restrict__ clouds) {
   const int32_t C = prm->num_cells;
-  const KeyMap m = load_keymap(prm);
-  const bool colidx = has_col_index(m, max_cols);
-  const int64_t ncols = colidx ? (1ll << m.bx) : 0;
-  const bool straddle = prm->kymin < 0 && prm->kymax >= 0;
-  const uint32_t yr0 = 0u - static_cast<uint32_t>(prm->kymin);  // y' of ky == 0
-  // cells of column x (x < 2^32): [lo, hi)
-  auto column = [&](int64_t x, int32_t& lo, int32_t& hi) {
-    if (colidx) {
-      if (x >= ncols) {
-        lo = hi = 0;
-      } else {
-        const int2 r = col_range[x];
-        lo = r.x;
-        hi = r.y;
-      }
-    } else {
-      lo = lower_bound_u64(cell_key, C, static_cast<uint64_t>(x) << 32);
-      hi = x == 0xFFFFFFFFll ? C
-                             : lower_bound_u64(cell_key, C, static_cast<uint64_t>(x + 1) << 32);
-    }
-  };
   for (int64_t c64 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c64 < C;
        c64 += static_cast<int64_t>(gridDim.x) * 256) {
     const int32_t c = static_cast<int32_t>(c64);
+    int32_t cb, ce, kymin, kymax;  // the cloud's cells [cb, ce) and y bounds
+    bool colidx;
+    int64_t col_base, ncols;
+    if constexpr (kSeg) {
+      const CloudInfo& ci = clouds[cell_cloud[c]];
+      cb = ci.cell_begin;
+      ce = ci.cell_end;
+      kymin = ci.kymin;
+      kymax = ci.kymax;
+      colidx = ci.col_base >= 0;
+      col_base = ci.col_base;
+      ncols = colidx ? (1ll << cloud_keymap(ci, 1).bx) : 0;
+    } else {
+      const KeyMap m = load_keymap(prm);
+      cb = 0;
+      ce = C;
+      kymin = prm->kymin;
+      kymax = prm->kymax;
+      colidx = has_col_index(m, max_cols);
+      col_base = 0;
+      ncols = colidx ? (1ll << m.bx) : 0;
+    }
+    // cells of column x (x < 2^32): [lo, hi)
+    auto column = [&](int64_t x, int32_t& lo, int32_t& hi) {
+      if (colidx) {
+        if (x >= ncols) {
+          lo = hi = 0;
+        } else {
+          const int2 r = col_range[col_base + x];
+          lo = r.x;
+          hi = r.y;
+        }
+      } else {
+        lo = cb + lower_bound_u64(cell_key + cb, ce - cb, static_cast<uint64_t>(x) << 32);
+        hi = x == 0xFFFFFFFFll
+                 ? ce
+                 : cb + lower_bound_u64(cell_key + cb, ce - cb, static_cast<uint64_t>(x + 1) << 32);
+      }
+    };
+    const bool straddle = kymin < 0 && kymax >= 0;
+    const uint32_t yr0 = 0u - static_cast<uint32_t>(kymin);  // y' of ky == 0
     const uint64_t key = cell_key[c];
     const uint32_t xr = static_cast<uint32_t>(key >> 32), yr = static_cast<uint32_t>(key);
     const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
@@ -803,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
                                                      (static_cast<uint64_t>(xr) << 32) | yr0);
       r = (yr >= yr0) ? own_lo + (c - split) : c + (own_hi - split);
     }
-    if (r < 0 || r >= C) {
+    if (r < cb || r >= ce) {
       atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
       r = c;
     }
@@ -867,6 +1138,7 @@ __device__ __forceinline__ int lowest_lane(uint64_t m) {
 // k_bbox; k_core_sparse sets the core bits of the other cells.
 constexpr uint32_t kCellHasCore = 1u, kCellHasNonCore = 2u, kCellHasNaN = 4u;
 
+template <class PP>
 __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               const float* __restrict__ xs_a,
                                               const float* __restrict__ ys_a,
@@ -877,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               uint4* __restrict__ cell_box,
                                               uint32_t* __restrict__ cell_flags,
                                               uint8_t* __restrict__ is_core, int64_t n,
-                                              int32_t min_samples) {
+                                              PP pp) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool in_a = (prm->npasses & 1) != 0;
@@ -890,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
   bool full = false;
   if (valid) {
     c = point_cell[i];
-    full = cell_start[c + 1] - cell_start[c] >= min_samples;
+    full = cell_start[c + 1] - cell_start[c] >= pp.at(c).ms;
     is_core[i] = full ? 1 : 0;  // points of other cells are decided by k_core_sparse
     const float x = xs[i], y = ys[i];
     if (!isnan(x) && !isnan(y)) {
@@ -1149,13 +1421,13 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
 // which cannot change the outcome.  Points whose count needs a scan of a big
 // cell, and cells of more than 64 points (possible when min_samples > 64),
 // are counted by the whole wave.  Sets the cell's core bits.
+template <class PP>
 __global__ __launch_bounds__(256) void k_core_sparse(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const float* __restrict__ xs_b,
     const float* __restrict__ ys_b, const int32_t* __restrict__ cell_start,
     const int32_t* __restrict__ ngh, const uint4* __restrict__ cell_box,
-    uint8_t* __restrict__ is_core, uint32_t* __restrict__ cell_flags, int32_t min_samples,
-    float eps_sq) {
+    uint8_t* __restrict__ is_core, uint32_t* __restrict__ cell_flags, PP pp) {
   const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
@@ -1167,10 +1439,14 @@ __global__ __launch_bounds__(256) void k_core_sparse(
     const int32_t c = static_cast<int32_t>(base + lane);
     const bool valid = base + lane < C;
     int32_t s = 0, e = 0;
+    CellParams cp{0.f, 0.f, 0};
     if (valid) {
       s = cell_start[c];
       e = cell_start[c + 1];
+      cp = pp.at(c);
     }
+    const int32_t min_samples = cp.ms;
+    const float eps_sq = cp.eps_sq;
     const bool sparse = valid && e - s < min_samples;
     const bool whole_wave = sparse && e - s > 64;
     uint64_t pending = 0;
@@ -1197,15 +1473,17 @@ __global__ __launch_bounds__(256) void k_core_sparse(
       const int src = lowest_lane(todo);
       todo &= todo - 1ull;
       const int32_t cc = bcast(c, src), cs = bcast(s, src), ce = bcast(e, src);
+      const int32_t wms = bcast(min_samples, src);
+      const float weps = bcastf(eps_sq, src);
       const uint64_t pm = bcast64(pending, src);
       const bool all = ce - cs > 64;
       const int32_t* row = ngh + static_cast<int64_t>(cc) * 10;
       for (int32_t p = cs; p < ce; ++p) {
         if (!all && !((pm >> (p - cs)) & 1ull)) continue;
         const int32_t tot = wave_core_count(xs, ys, xs[p], ys[p], cc, row, cell_start,
-                                            cell_flags, cell_box, min_samples, eps_sq, ce - cs);
+                                            cell_flags, cell_box, wms, weps, ce - cs);
         if (lane == src) {
-          const bool core = tot >= min_samples;
+          const bool core = tot >= wms;
           is_core[p] = core ? 1 : 0;
           bits |= core ? kCellHasCore : kCellHasNonCore;
         }
@@ -1325,14 +1603,14 @@ __device__ __forceinline__ int32_t forward_neighbour(const int32_t* __restrict__
   return row[8] + (slot - 7) < row[9] ? row[8] + (slot - 7) : -1;
 }
 
-template <int kPhase>
+template <int kPhase, class PP>
 __global__ __launch_bounds__(256) void k_union(
     Params* __restrict__ prm, const float* __restrict__ xs_a, const float* __restrict__ ys_a,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b,
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
     const int32_t* __restrict__ cell_ref, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const uint64_t* __restrict__ cell_key,
-    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, float eps_sq) {
+    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, PP pp) {
   constexpr int S = kPhase == 0 ? 7 : kUnionSlots;  // slots scanned by this phase
   const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
@@ -1362,7 +1640,9 @@ __global__ __launch_bounds__(256) void k_union(
     if (nc >= 0 && !(cell_flags[nc] & kCellHasCore)) nc = -1;
     if (kPhase == 1 && nc >= 0 && uf[cell_ref[c]] == uf[cell_ref[nc]]) nc = -1;
     bool deferred = false;
+    float eps_sq = 0.f;  // c and nc are cells of one cloud
     if (nc >= 0) {
+      eps_sq = pp.at(c).eps_sq;
       const float4 box_c = load_box(cell_box, c), box_n = load_box(cell_box, nc);
       if (box_box_d2(box_c, box_n) <= eps_sq) {
         const int32_t sa = cell_start[c], ea = cell_start[c + 1];
@@ -1380,11 +1660,12 @@ __global__ __launch_bounds__(256) void k_union(
       const int src = lowest_lane(todo);
       todo &= todo - 1ull;
       const int32_t cw = bcast(c, src), nw = bcast(nc, src);
+      const float weps = bcastf(eps_sq, src);
       const int32_t rc = cell_ref[cw], rn = cell_ref[nw];
       if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
       const bool hit = core_pair_within(xs, ys, is_core, cell_start[cw], cell_start[cw + 1],
                                         load_box(cell_box, cw), cell_start[nw],
-                                        cell_start[nw + 1], load_box(cell_box, nw), eps_sq);
+                                        cell_start[nw + 1], load_box(cell_box, nw), weps);
       if (hit && lane == 0) uf_unite(uf, rc, rn, &prm->error);
     }
   }
@@ -1442,16 +1723,26 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
   }
 }
 
+// kSeg (batches): ids count from 0 in every cloud, so the number of roots
+// before the cloud's first reference index is subtracted.
+template <bool kSeg>
 __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
                                                  const int32_t* __restrict__ cell_ref,
                                                  const uint32_t* __restrict__ cell_flags,
                                                  const int32_t* __restrict__ uf,
                                                  const uint32_t* __restrict__ rank,
-                                                 int32_t* __restrict__ cell_cluster) {
+                                                 int32_t* __restrict__ cell_cluster,
+                                                 const int32_t* __restrict__ cell_cloud,
+                                                 const CloudInfo* __restrict__ clouds) {
   const int32_t C = prm->num_cells;
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
        c += static_cast<int64_t>(gridDim.x) * 256) {
-    cell_cluster[c] = (cell_flags[c] & 1u) ? static_cast<int32_t>(rank[uf[cell_ref[c]]]) : kNoise;
+    int32_t id = kNoise;
+    if (cell_flags[c] & 1u) {
+      id = static_cast<int32_t>(rank[uf[cell_ref[c]]]);
+      if constexpr (kSeg) id -= static_cast<int32_t>(rank[clouds[cell_cloud[c]].cell_begin]);
+    }
+    cell_cluster[c] = id;
   }
 }
 
@@ -1554,6 +1845,7 @@ __device__ int32_t wave_border(const float* __restrict__ xs, const float* __rest
 // in turn and writes the label to the point's input position.  Points whose
 // search needs a big cell, and cells of more than 64 points, are searched by
 // the whole wave.
+template <class PP>
 __global__ __launch_bounds__(256) void k_border(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const uint32_t* __restrict__ idx_a,
@@ -1561,8 +1853,7 @@ __global__ __launch_bounds__(256) void k_border(
     const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ cell_start,
     const int32_t* __restrict__ ngh, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const uint8_t* __restrict__ is_core,
-    const int32_t* __restrict__ cell_cluster, int32_t* __restrict__ labels, int64_t n,
-    float eps_sq, float best_init) {
+    const int32_t* __restrict__ cell_cluster, int32_t* __restrict__ labels, int64_t n, PP pp) {
   const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
@@ -1576,11 +1867,14 @@ __global__ __launch_bounds__(256) void k_border(
     const bool valid = base + lane < C;
     int32_t s = 0, e = 0;
     bool has = false;
+    CellParams cp{0.f, 0.f, 0};
     if (valid && (cell_flags[c] & kCellHasNonCore)) {
       has = true;
       s = cell_start[c];
       e = cell_start[c + 1];
+      cp = pp.at(c);
     }
+    const float eps_sq = cp.eps_sq, best_init = cp.best_init;
     const bool whole_wave = has && e - s > 64;
     uint64_t pending = 0;
     if (has && !whole_wave) {
@@ -1603,13 +1897,14 @@ __global__ __launch_bounds__(256) void k_border(
       const int src = lowest_lane(todo);
       todo &= todo - 1ull;
       const int32_t cc = bcast(c, src), cs = bcast(s, src), ce = bcast(e, src);
+      const float weps = bcastf(eps_sq, src), wbest = bcastf(best_init, src);
       const uint64_t pm = bcast64(pending, src);
       const bool all = ce - cs > 64;
       const int32_t* row = ngh + static_cast<int64_t>(cc) * 10;
       for (int32_t p = cs; p < ce; ++p) {
         if (all ? is_core[p] != 0 : !((pm >> (p - cs)) & 1ull)) continue;
         const int32_t lab = wave_border(xs, ys, is_core, xs[p], ys[p], row, cell_start,
-                                        cell_flags, cell_box, cell_cluster, eps_sq, best_init);
+                                        cell_flags, cell_box, cell_cluster, weps, wbest);
         if (lane == src) {
           const uint32_t dst = idx[p];
           if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
@@ -1623,22 +1918,25 @@ __global__ __launch_bounds__(256) void k_border(
 // Host side
 // ---------------------------------------------------------------------------
 
+// Workspace layout for n points.  clouds > 0: a batch of that many clouds
+// holding n points in all (tile counts are bounded by n / tile + clouds).
 struct Layout {
-  size_t params, sort_status, cell_status, rank_status, zero_end;
-  size_t partials;
+  size_t params, sort_status, cell_status, rank_status, cloud_hist, zero_end;
+  size_t partials, clouds, seg_sort, seg_scan, seg_end;
   size_t xs_a, ys_a, idx_a, xs_b, ys_b, idx_b;
   size_t point_cell, is_core, cell_start, cell_key, cell_flags, cell_box, ngh, cell_ref, uf, rank,
-      cell_cluster, col_range;
+      cell_cluster, col_range, cell_cloud;
   int64_t max_cols;
   size_t total;
 };
 
 constexpr size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
-Layout make_layout(int64_t n) {
+Layout make_layout(int64_t n, int64_t clouds = 0) {
   const size_t N = static_cast<size_t>(n > 0 ? n : 1);
-  const size_t sort_tiles = (N + kSortTile - 1) / kSortTile;
-  const size_t scan_tiles = (N + kScanTile - 1) / kScanTile;
+  const size_t B = static_cast<size_t>(clouds > 0 ? clouds : 0);
+  const size_t sort_tiles = (N + kSortTile - 1) / kSortTile + B;
+  const size_t scan_tiles = (N + kScanTile - 1) / kScanTile + B;
   Layout L{};
   size_t off = 0;
   auto take = [&off](size_t bytes) {
@@ -1650,8 +1948,13 @@ Layout make_layout(int64_t n) {
   L.sort_status = take(sort_tiles * kRadixMax * sizeof(uint64_t));
   L.cell_status = take(scan_tiles * sizeof(uint64_t));
   L.rank_status = take(scan_tiles * sizeof(uint64_t));
+  L.cloud_hist = take(B * kMaxPasses * kRadixMax * sizeof(uint32_t));
   L.zero_end = off;
   L.partials = take(kMaxReduceBlocks * sizeof(int4));
+  L.clouds = take(B * sizeof(CloudInfo));
+  L.seg_sort = take(B ? sort_tiles * sizeof(SegTile) : 0);
+  L.seg_scan = take(B ? scan_tiles * sizeof(SegTile) : 0);
+  L.seg_end = off;
   L.xs_a = take(N * 4);
   L.ys_a = take(N * 4);
   L.idx_a = take(N * 4);
@@ -1671,6 +1974,7 @@ Layout make_layout(int64_t n) {
   L.cell_cluster = take(N * 4);
   L.max_cols = static_cast<int64_t>(std::min<size_t>(size_t(1) << 22, 4 * N + 64));
   L.col_range = take(static_cast<size_t>(L.max_cols) * sizeof(int2));
+  L.cell_cloud = take(B ? N * 4 : 0);
   L.total = off;
   return L;
 }
@@ -1749,6 +2053,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const float inv_r = static_cast<float>(std::sqrt(2.0)) / eps_f;
   const float eps_sq = eps_f * eps_f;
   const float best_init = eps_sq + 1.0f;
+  const OneCloud pp{eps_sq, best_init, ms};
 
   hipEvent_t ev[DBSCAN_HIP_NUM_STAGES + 1] = {};
   auto release = [&]() {
@@ -1831,47 +2136,47 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   // Stage 2: radix passes (sort + gather in one).
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  k_sort_pass<true><<<g_sort, kSortThreads, 0, stream>>>(X2, nullptr, nullptr, nullptr, xs_a,
-                                                         ys_a, idx_a, prm, sort_status, 0, n,
-                                                         inv_r);
+  k_sort_pass<true, false><<<g_sort, kSortThreads, 0, stream>>>(
+      X2, nullptr, nullptr, nullptr, xs_a, ys_a, idx_a, prm, sort_status, 0, n, inv_r, nullptr,
+      nullptr, nullptr);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    k_sort_pass<false><<<g_sort, kSortThreads, 0, stream>>>(
+    k_sort_pass<false, false><<<g_sort, kSortThreads, 0, stream>>>(
         nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
         from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, sort_status, p,
-        n, inv_r);
+        n, inv_r, nullptr, nullptr, nullptr);
   }
 
   // Stage 3: cell runs.
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
-  k_cells<<<g_scan, kScanThreads, 0, stream>>>(xs_a, ys_a, xs_b, ys_b, prm,
-                                               at<uint64_t>(ws, L.cell_status), point_cell,
-                                               cell_start, cell_key, cell_flags, cell_box,
-                                               at<int2>(ws, L.col_range), L.max_cols, n, inv_r);
+  k_cells<false><<<g_scan, kScanThreads, 0, stream>>>(
+      xs_a, ys_a, xs_b, ys_b, prm, at<uint64_t>(ws, L.cell_status), point_cell, cell_start,
+      cell_key, cell_flags, cell_box, at<int2>(ws, L.col_range), L.max_cols, n, inv_r, nullptr,
+      nullptr, nullptr);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
   k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
-                                      cell_box, cell_flags, is_core, n, ms);
+                                      cell_box, cell_flags, is_core, n, pp);
 
   // Stage 4: window columns.
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
-  k_ngh<<<g_cells, 256, 0, stream>>>(prm, cell_key, at<int2>(ws, L.col_range), L.max_cols, ngh,
-                                     cell_ref, uf);
+  k_ngh<false><<<g_cells, 256, 0, stream>>>(prm, cell_key, at<int2>(ws, L.col_range),
+                                            L.max_cols, ngh, cell_ref, uf, nullptr, nullptr);
 
   // Stage 5: core marking.
   if ((e = before(kKCore)) != hipSuccess) return fail(e);
   k_core_sparse<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh,
-                                                 cell_box, is_core, cell_flags, ms, eps_sq);
+                                                 cell_box, is_core, cell_flags, pp);
 
   // Stage 6: union-find.
   if ((e = before(kKUnion)) != hipSuccess) return fail(e);
   k_union<0><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                          cell_flags, cell_box, cell_key, is_core, uf, eps_sq);
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
   if ((e = before(kKFlatten0)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
   if ((e = before(kKUnion1)) != hipSuccess) return fail(e);
   k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                          cell_flags, cell_box, cell_key, is_core, uf, eps_sq);
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
   if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
 
@@ -1879,7 +2184,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   if ((e = before(kKRank)) != hipSuccess) return fail(e);
   k_rank<<<g_scan, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank);
   if ((e = before(kKCluster)) != hipSuccess) return fail(e);
-  k_cluster<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank, cell_cluster);
+  k_cluster<false><<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank,
+                                                cell_cluster, nullptr, nullptr);
 
   // Stages 8-10: core labels, border assignment and unpermute, fused.
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
@@ -1888,7 +2194,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   if ((e = before(kKBorder)) != hipSuccess) return fail(e);
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
                                             ngh, cell_flags, cell_box, is_core, cell_cluster,
-                                            labels, n, eps_sq, best_init);
+                                            labels, n, pp);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
@@ -1899,6 +2205,191 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
     }
   }
   release();
+  return DBSCAN_HIP_OK;
+}
+
+// Per-cloud parameters as a single call computes them (see run()).
+bool cloud_params(double eps, int64_t min_samples64, CloudInfo& ci) {
+  if (!(eps > 0.0) || min_samples64 < 1) return false;
+  const float eps_f = static_cast<float>(eps);
+  const int32_t ms =
+      static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(min_samples64)));
+  ci.inv_r = static_cast<float>(std::sqrt(2.0)) / eps_f;
+  ci.eps_sq = eps_f * eps_f;
+  ci.best_init = ci.eps_sq + 1.0f;
+  // a narrowed min_samples below 1 makes every point noise
+  // (dbscan_grid.hpp:141); no cell of a batch reaches INT32_MAX points
+  ci.ms = ms < 1 ? INT32_MAX : ms;
+  return true;
+}
+
+int run_batched(const float* X, int64_t total, const int64_t* offsets, const double* eps,
+                const int64_t* min_samples, int64_t num_clouds, int32_t* labels,
+                void* workspace, size_t workspace_bytes, hipStream_t stream, hipEvent_t* kev,
+                int* bad_cloud) {
+  if (bad_cloud) *bad_cloud = -1;
+  if (num_clouds < 0 || num_clouds > (1 << 24)) return DBSCAN_HIP_ERR_N;
+  if (total < 0 || total >= INT32_MAX) return DBSCAN_HIP_ERR_N;
+  if (num_clouds == 0) return total == 0 ? DBSCAN_HIP_OK : DBSCAN_HIP_ERR_N;
+  if (!offsets || !eps || !min_samples) return DBSCAN_HIP_ERR_NULL;
+  const int32_t B = static_cast<int32_t>(num_clouds);
+  std::vector<CloudInfo> info(B);
+  int64_t sort_tiles = 0, scan_tiles = 0;
+  if (offsets[0] != 0 || offsets[B] != total) return DBSCAN_HIP_ERR_N;
+  for (int32_t i = 0; i < B; ++i) {
+    CloudInfo& ci = info[i];
+    std::memset(&ci, 0, sizeof(ci));
+    if (offsets[i + 1] < offsets[i]) {
+      if (bad_cloud) *bad_cloud = i;
+      return DBSCAN_HIP_ERR_N;
+    }
+    if (!cloud_params(eps[i], min_samples[i], ci)) {
+      if (bad_cloud) *bad_cloud = i;
+      return !(eps[i] > 0.0) ? DBSCAN_HIP_ERR_EPS : DBSCAN_HIP_ERR_MIN_SAMPLES;
+    }
+    ci.begin = static_cast<int32_t>(offsets[i]);
+    ci.end = static_cast<int32_t>(offsets[i + 1]);
+    ci.kxmin = INT_MAX; ci.kxmax = INT_MIN; ci.kymin = INT_MAX; ci.kymax = INT_MIN;
+    ci.col_base = -1;
+    const int64_t ni = offsets[i + 1] - offsets[i];
+    sort_tiles += (ni + kSortTile - 1) / kSortTile;
+    scan_tiles += (ni + kScanTile - 1) / kScanTile;
+  }
+  if (total == 0) return DBSCAN_HIP_OK;
+  if (!X || !labels || !workspace) return DBSCAN_HIP_ERR_NULL;
+  if ((reinterpret_cast<uintptr_t>(X) & 7u) || (reinterpret_cast<uintptr_t>(labels) & 3u) ||
+      (reinterpret_cast<uintptr_t>(workspace) & 255u))
+    return DBSCAN_HIP_ERR_ALIGN;
+  const Layout L = make_layout(total, num_clouds);
+  if (workspace_bytes < L.total) return DBSCAN_HIP_ERR_WORKSPACE;
+
+  // Cloud records and tile tables, staged on the host and copied in one go
+  // (the copy is ordered before the kernels on `stream`).
+  std::vector<char> staging(L.seg_end - L.clouds, 0);
+  std::memcpy(staging.data(), info.data(), sizeof(CloudInfo) * B);
+  SegTile* st_sort = reinterpret_cast<SegTile*>(staging.data() + (L.seg_sort - L.clouds));
+  SegTile* st_scan = reinterpret_cast<SegTile*>(staging.data() + (L.seg_scan - L.clouds));
+  {
+    int32_t a = 0, b = 0;
+    for (int32_t i = 0; i < B; ++i) {
+      const int32_t first = a;
+      for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kSortTile)
+        st_sort[a++] = SegTile{i, p0, std::min(kSortTile, info[i].end - p0), first};
+      for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kScanTile)
+        st_scan[b++] = SegTile{i, p0, std::min(kScanTile, info[i].end - p0), 0};
+    }
+  }
+
+  auto fail = [&](hipError_t e) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  };
+  auto before = [&](int k) -> hipError_t {
+    return kev ? hipEventRecord(kev[k], stream) : hipSuccess;
+  };
+  hipError_t e;
+  void* ws = workspace;
+  if ((e = hipMemcpyAsync(at<char>(ws, L.clouds), staging.data(), staging.size(),
+                          hipMemcpyHostToDevice, stream)) != hipSuccess)
+    return fail(e);
+
+  const int64_t n = total;
+  Params* prm = at<Params>(ws, L.params);
+  uint64_t* sort_status = at<uint64_t>(ws, L.sort_status);
+  uint32_t* cloud_hist = at<uint32_t>(ws, L.cloud_hist);
+  CloudInfo* clouds = at<CloudInfo>(ws, L.clouds);
+  const SegTile* seg_sort = at<SegTile>(ws, L.seg_sort);
+  const SegTile* seg_scan = at<SegTile>(ws, L.seg_scan);
+  const float2* X2 = reinterpret_cast<const float2*>(X);
+  float* xs_a = at<float>(ws, L.xs_a);
+  float* ys_a = at<float>(ws, L.ys_a);
+  uint32_t* idx_a = at<uint32_t>(ws, L.idx_a);
+  float* xs_b = at<float>(ws, L.xs_b);
+  float* ys_b = at<float>(ws, L.ys_b);
+  uint32_t* idx_b = at<uint32_t>(ws, L.idx_b);
+  int32_t* point_cell = at<int32_t>(ws, L.point_cell);
+  uint8_t* is_core = at<uint8_t>(ws, L.is_core);
+  int32_t* cell_start = at<int32_t>(ws, L.cell_start);
+  uint64_t* cell_key = at<uint64_t>(ws, L.cell_key);
+  uint32_t* cell_flags = at<uint32_t>(ws, L.cell_flags);
+  uint4* cell_box = at<uint4>(ws, L.cell_box);
+  int32_t* ngh = at<int32_t>(ws, L.ngh);
+  int32_t* cell_ref = at<int32_t>(ws, L.cell_ref);
+  int32_t* uf = at<int32_t>(ws, L.uf);
+  uint32_t* rank = at<uint32_t>(ws, L.rank);
+  int32_t* cell_cluster = at<int32_t>(ws, L.cell_cluster);
+  int32_t* cell_cloud = at<int32_t>(ws, L.cell_cloud);
+  int2* col_range = at<int2>(ws, L.col_range);
+  const ManyClouds pp{cell_cloud, clouds};
+
+  const int g_sort = static_cast<int>(sort_tiles);
+  const int g_scan_pts = static_cast<int>(scan_tiles);
+  const int g_scan_cells = grid_for(n, kScanTile, INT_MAX);
+  const int g_point = grid_for(n, 256, INT_MAX);
+  const int g_cells = grid_for(n, 256, 16384);
+  const int g_cellwaves = grid_for(n, 256, 8192);
+  const int g_union = grid_for(n * 7, 256, 16384);
+  const int hist_per_block = static_cast<int>((sort_tiles + 1023) / 1024);
+  const int g_hist = static_cast<int>((sort_tiles + hist_per_block - 1) / hist_per_block);
+
+  if ((e = before(kKBounds)) != hipSuccess) return fail(e);
+  const int64_t zero16 = static_cast<int64_t>((L.zero_end - kParamsZeroBegin) / 16);
+  k_bounds_seg<<<g_sort, 256, 0, stream>>>(X2, seg_sort, clouds, at<uint4>(ws, kParamsZeroBegin),
+                                           zero16);
+  k_keys_seg<<<1, 1024, 0, stream>>>(prm, clouds, B, L.max_cols);
+  if ((e = before(kKHist)) != hipSuccess) return fail(e);
+  k_hist_seg<<<g_hist, 256, 0, stream>>>(X2, seg_sort, g_sort, hist_per_block, clouds, prm,
+                                         cloud_hist, col_range);
+  if ((e = before(kKSort0)) != hipSuccess) return fail(e);
+  k_sort_pass<true, true><<<g_sort, kSortThreads, 0, stream>>>(
+      X2, nullptr, nullptr, nullptr, xs_a, ys_a, idx_a, prm, sort_status, 0, n, 0.f, seg_sort,
+      clouds, cloud_hist);
+  for (int p = 1; p < kMaxPasses; ++p) {
+    if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
+    const bool from_a = (p & 1) != 0;
+    k_sort_pass<false, true><<<g_sort, kSortThreads, 0, stream>>>(
+        nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
+        from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, sort_status, p,
+        n, 0.f, seg_sort, clouds, cloud_hist);
+  }
+  if ((e = before(kKCells)) != hipSuccess) return fail(e);
+  k_cells<true><<<g_scan_pts, kScanThreads, 0, stream>>>(
+      xs_a, ys_a, xs_b, ys_b, prm, at<uint64_t>(ws, L.cell_status), point_cell, cell_start,
+      cell_key, cell_flags, cell_box, col_range, L.max_cols, n, 0.f, seg_scan, clouds,
+      cell_cloud);
+  if ((e = before(kKBbox)) != hipSuccess) return fail(e);
+  k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
+                                      cell_box, cell_flags, is_core, n, pp);
+  if ((e = before(kKNgh)) != hipSuccess) return fail(e);
+  k_ngh<true><<<g_cells, 256, 0, stream>>>(prm, cell_key, col_range, L.max_cols, ngh, cell_ref,
+                                           uf, cell_cloud, clouds);
+  if ((e = before(kKCore)) != hipSuccess) return fail(e);
+  k_core_sparse<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh,
+                                                 cell_box, is_core, cell_flags, pp);
+  if ((e = before(kKUnion)) != hipSuccess) return fail(e);
+  k_union<0><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
+  if ((e = before(kKFlatten0)) != hipSuccess) return fail(e);
+  k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
+  if ((e = before(kKUnion1)) != hipSuccess) return fail(e);
+  k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
+  if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
+  k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
+  if ((e = before(kKRank)) != hipSuccess) return fail(e);
+  k_rank<<<g_scan_cells, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank);
+  if ((e = before(kKCluster)) != hipSuccess) return fail(e);
+  k_cluster<true><<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank, cell_cluster,
+                                               cell_cloud, clouds);
+  if ((e = before(kKLabel)) != hipSuccess) return fail(e);
+  k_label<<<g_point, 256, 0, stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster,
+                                       labels, n);
+  if ((e = before(kKBorder)) != hipSuccess) return fail(e);
+  k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
+                                            ngh, cell_flags, cell_box, is_core, cell_cluster,
+                                            labels, n, pp);
+  if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   return DBSCAN_HIP_OK;
 }
 
@@ -1941,6 +2432,33 @@ int dbscan2d_hip_profile(const float* X, int64_t n, double eps, int64_t min_samp
 
 const char* dbscan2d_hip_stage_name(int i) {
   return (i >= 0 && i < DBSCAN_HIP_NUM_STAGES) ? kStageNames[i] : nullptr;
+}
+
+int dbscan2d_hip_batched_workspace_bytes(int64_t total_points, int64_t num_clouds,
+                                         size_t* bytes) {
+  if (!bytes) return DBSCAN_HIP_ERR_NULL;
+  if (total_points < 0 || total_points >= INT32_MAX || num_clouds < 0 ||
+      num_clouds > (1 << 24))
+    return DBSCAN_HIP_ERR_N;
+  *bytes = make_layout(total_points, num_clouds > 0 ? num_clouds : 1).total;
+  return DBSCAN_HIP_OK;
+}
+
+int dbscan2d_hip_batched(const float* X, int64_t total_points, const int64_t* offsets,
+                         const double* eps, const int64_t* min_samples, int64_t num_clouds,
+                         int32_t* labels, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream, int* bad_cloud) {
+  return run_batched(X, total_points, offsets, eps, min_samples, num_clouds, labels, workspace,
+                     workspace_bytes, stream, nullptr, bad_cloud);
+}
+
+int dbscan2d_hip_batched_timed(const float* X, int64_t total_points, const int64_t* offsets,
+                               const double* eps, const int64_t* min_samples,
+                               int64_t num_clouds, int32_t* labels, void* workspace,
+                               size_t workspace_bytes, hipStream_t stream, hipEvent_t* events) {
+  if (!events) return DBSCAN_HIP_ERR_NULL;
+  return run_batched(X, total_points, offsets, eps, min_samples, num_clouds, labels, workspace,
+                     workspace_bytes, stream, events, nullptr);
 }
 
 int dbscan2d_hip_layout(int64_t n, dbscan_hip_layout* out) {

@@ -10,7 +10,7 @@ bit-identical to the reference's CPU path.
 
 from importlib.metadata import PackageNotFoundError
 from importlib.metadata import version as _pkg_version
-from typing import Any, Optional, Union
+from typing import Any, Optional, Sequence, Union
 
 import numpy as np
 import torch
@@ -43,7 +43,7 @@ except ImportError as exc:  # the native library is the product: fail loudly
 
 ArrayLike2D = Union[np.ndarray, torch.Tensor]
 
-__all__ = ["dbscan2d", "DBSCAN", "ArrayLike2D"]
+__all__ = ["dbscan2d", "dbscan2d_batched", "DBSCAN", "ArrayLike2D"]
 
 try:
     __version__ = _pkg_version("dbscan-torch")
@@ -68,6 +68,63 @@ def dbscan2d(X: torch.Tensor, eps: float, min_samples: int) -> torch.Tensor:
         in ``[0, n_clusters)``, numbered as the reference CPU path numbers them.
     """
     return _C.dbscan2d(X, float(eps), int(min_samples))
+
+
+def _per_cloud(value, count: int, name: str, cast):
+    if isinstance(value, (list, tuple, np.ndarray, torch.Tensor)):
+        vals = [cast(v) for v in (value.tolist() if hasattr(value, "tolist") else value)]
+        if len(vals) != count:
+            raise ValueError(f"dbscan2d_batched: {name} has {len(vals)} values for {count} clouds")
+        return vals
+    return [cast(value)] * count
+
+
+def dbscan2d_batched(clouds: Union[Sequence[torch.Tensor], torch.Tensor],
+                     eps: Union[float, Sequence[float]],
+                     min_samples: Union[int, Sequence[int]],
+                     offsets: Optional[Sequence[int]] = None):
+    """Cluster many independent 2-D point clouds in one batched launch.
+
+    No counterpart in the reference (which clusters one cloud per call); every
+    cloud's labels are exactly those ``dbscan2d(cloud, eps_i, min_samples_i)``
+    returns, numbered from 0 within the cloud.
+
+    Args:
+        clouds: a sequence of float32 ``(N_i, 2)`` tensors (all on one device),
+            or one packed ``(N, 2)`` tensor together with ``offsets``.
+        eps: one radius for every cloud, or one per cloud (each > 0).
+        min_samples: one value for every cloud, or one per cloud (each >= 1).
+        offsets: with a packed tensor, ``B + 1`` row offsets from 0 to N; cloud
+            i is rows ``[offsets[i], offsets[i+1])``.
+
+    Returns:
+        With a sequence: a list of int32 label tensors, one per cloud, on the
+        input device.  With a packed tensor: the packed int32 labels ``(N,)``.
+    """
+    if isinstance(clouds, torch.Tensor):
+        if offsets is None:
+            raise ValueError("dbscan2d_batched: a packed tensor needs offsets")
+        offs = [int(v) for v in (offsets.tolist() if hasattr(offsets, "tolist") else offsets)]
+        B = len(offs) - 1
+        return _C.dbscan2d_batched(clouds, offs, _per_cloud(eps, B, "eps", float),
+                                   _per_cloud(min_samples, B, "min_samples", int))
+    parts = list(clouds)
+    B = len(parts)
+    if B == 0:
+        return []
+    for p in parts:
+        if not (isinstance(p, torch.Tensor) and p.dim() == 2 and p.size(1) == 2):
+            raise RuntimeError("X must have shape (N, 2)")
+        if p.dtype != torch.float32:
+            raise RuntimeError("X must be float32")
+    sizes = [int(p.size(0)) for p in parts]
+    offs = [0]
+    for n in sizes:
+        offs.append(offs[-1] + n)
+    packed = torch.cat(parts, 0) if B > 1 else parts[0]
+    labels = _C.dbscan2d_batched(packed, offs, _per_cloud(eps, B, "eps", float),
+                                 _per_cloud(min_samples, B, "min_samples", int))
+    return list(torch.split(labels, sizes))
 
 
 class DBSCAN:

@@ -95,6 +95,41 @@ int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_sample
                        hipStream_t stream, hipEvent_t* events);
 
 /*
+ * Batched form: num_clouds independent clouds packed one after another in X
+ * (total_points points in all, cloud i being points [offsets[i],
+ * offsets[i+1])), each with its own eps[i] and min_samples[i], clustered in
+ * one pipeline of launches.  labels (total_points int32, device memory) gets,
+ * for every cloud, exactly the labels dbscan2d_hip would give that cloud
+ * alone, numbered from 0 within the cloud.  offsets (num_clouds + 1 values,
+ * offsets[0] == 0, non-decreasing, offsets[num_clouds] == total_points), eps
+ * and min_samples are HOST arrays, validated per cloud as dbscan2d_hip
+ * validates its arguments; on a per-cloud error *bad_cloud (if not NULL)
+ * names the cloud.  total_points < INT32_MAX.  The per-cloud records and tile
+ * tables are copied into the workspace with hipMemcpyAsync on `stream`.
+ *
+ * No counterpart in the reference, whose binding clusters one cloud per call
+ * (csrc/extension.cpp:33-45); this is the batched launch that replaces a
+ * Python loop of dbscan2d calls over many clouds.
+ */
+int dbscan2d_hip_batched_workspace_bytes(int64_t total_points, int64_t num_clouds,
+                                         size_t* bytes);
+int dbscan2d_hip_batched(const float* X, int64_t total_points, const int64_t* offsets,
+                         const double* eps, const int64_t* min_samples, int64_t num_clouds,
+                         int32_t* labels, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream, int* bad_cloud);
+
+/*
+ * dbscan2d_hip_batched with the event slots of dbscan2d_hip_timed (slot 0
+ * also covers the per-cloud key-map kernel).  No counterpart in the
+ * reference: per-kernel timing of batches.
+ */
+int dbscan2d_hip_batched_timed(const float* X, int64_t total_points, const int64_t* offsets,
+                               const double* eps, const int64_t* min_samples,
+                               int64_t num_clouds, int32_t* labels, void* workspace,
+                               size_t workspace_bytes, hipStream_t stream,
+                               hipEvent_t* events);
+
+/*
  * Byte offsets, inside the workspace of a dbscan2d_hip call on n points, of
  * the intermediates that call leaves behind (all in the library's sorted
  * point order and internal cell order).  For stage-wise comparison with the
@@ -106,7 +141,7 @@ int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_sample
 typedef struct {
   size_t info;        /* int32[8]: kxmin, kxmax, kymin, kymax, num_cells, npasses,
                          error bits (0 when every bounded wait and index check
-                         passed), 0 */
+                         passed), column-index entries of a batch (0) */
   size_t xs_a, ys_a, idx_a;   /* sorted x, y, original index if npasses is odd  */
   size_t xs_b, ys_b, idx_b;   /* ... if npasses is even                         */
   size_t point_cell;  /* int32[n]  cell of each sorted point                      */

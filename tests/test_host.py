@@ -122,3 +122,60 @@ def test_class_wrapper_contract_without_compute():
     assert (clf.eps, clf.min_samples, clf.labels_) == (0.5, 5, None)
     out = DBSCAN(eps=1.0, min_samples=3).fit(np.zeros((0, 2), np.float32))
     assert isinstance(out.labels_, np.ndarray) and out.labels_.shape == (0,)
+
+
+def test_batched_validation_messages():
+    from dbscan_torch import dbscan2d_batched
+
+    a, b = torch.zeros((4, 2)), torch.zeros((3, 2))
+    with pytest.raises(RuntimeError, match=r"eps must be > 0 \(cloud 1\)"):
+        dbscan2d_batched([a, b], [0.5, 0.0], 5)
+    with pytest.raises(RuntimeError, match=r"min_samples must be >= 1 \(cloud 0\)"):
+        dbscan2d_batched([a, b], 0.5, [0, 5])
+    with pytest.raises(RuntimeError, match=r"shape \(N, 2\)"):
+        dbscan2d_batched([a, torch.zeros((3, 3))], 0.5, 5)
+    with pytest.raises(RuntimeError, match="float32"):
+        dbscan2d_batched([a, b.double()], 0.5, 5)
+    with pytest.raises(ValueError, match="2 values for 3 clouds"):
+        dbscan2d_batched([a, b, a], [0.5, 0.5], 5)
+    packed = torch.zeros((7, 2))
+    with pytest.raises(RuntimeError, match="offsets must run from 0 to N"):
+        dbscan2d_batched(packed, 0.5, 5, offsets=[0, 4, 6])
+    with pytest.raises(RuntimeError, match="non-decreasing"):
+        dbscan2d_batched(packed, 0.5, 5, offsets=[0, 5, 4, 7])
+    assert dbscan2d_batched([], 0.5, 5) == []
+    out = dbscan2d_batched(torch.zeros((0, 2)), 0.5, 5, offsets=[0, 0, 0])
+    assert out.shape == (0,) and out.dtype == torch.int32
+
+
+def test_batched_c_abi_argument_checks():
+    lib = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    lib.dbscan2d_hip_batched.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int64, P, P,
+                                         ctypes.c_size_t, P, ctypes.POINTER(ctypes.c_int)]
+    lib.dbscan2d_hip_batched_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int64,
+                                                         ctypes.POINTER(ctypes.c_size_t)]
+    offs = np.array([0, 6, 6, 10], dtype=np.int64)
+    eps = np.array([0.5, 1.0, -1.0], dtype=np.float64)
+    ms = np.array([5, 0, 5], dtype=np.int64)
+    bad = ctypes.c_int(0)
+
+    def call(o, e, m, total=10, B=3, ws=0x3000, ws_bytes=1 << 30):
+        return lib.dbscan2d_hip_batched(0x1000, total, o.ctypes.data, e.ctypes.data,
+                                        m.ctypes.data, B, 0x2000, ws, ws_bytes, None,
+                                        ctypes.byref(bad))
+
+    assert call(offs, eps, ms) == 3 and bad.value == 1          # min_samples of cloud 1
+    ms[1] = 5
+    assert call(offs, eps, ms) == 2 and bad.value == 2          # eps of cloud 2
+    eps[2] = 2.0
+    assert call(offs, eps, ms, total=11) == 1                   # offsets do not end at N
+    assert call(np.array([0, 7, 6, 10], dtype=np.int64), eps, ms) == 1 and bad.value == 1
+    assert call(offs, eps, ms, ws=0x3001) == 6                  # misaligned workspace
+    assert call(offs, eps, ms, ws_bytes=16) == 5                # workspace too small
+    sizes = []
+    for total, B in ((10, 1), (10, 3), (1_000_000, 1), (1_000_000, 2048)):
+        b = ctypes.c_size_t(0)
+        assert lib.dbscan2d_hip_batched_workspace_bytes(total, B, ctypes.byref(b)) == 0
+        sizes.append(b.value)
+    assert sizes[0] <= sizes[1] < sizes[2] < sizes[3]
