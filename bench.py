@@ -16,6 +16,9 @@ min_samples 10, seed 2).  Prints one JSON line:
   cpu_baseline   the reference's own CPU path (compiled from its sources into
                  oracle/_ref) on the same workload, on the host cores, or the C
                  oracle where that build is absent
+  batched        2,048 clouds (~135M points, per-cloud eps and min_samples):
+                 clouds/s of one batched call against a Python loop of plain
+                 dbscan2d calls over the same resident clouds, labels compared
 """
 
 from __future__ import annotations
@@ -103,6 +106,66 @@ def cpu_baseline(X, eps, ms, reps=3):
     }
 
 
+def batched_leg(dbscan_torch, steps: int, warmup: int) -> dict:
+    """The 2,048-cloud workload (per-cloud N log-uniform in [200, 500,000],
+    ~135M points, own eps and min_samples per cloud): one batched call per
+    step against a Python loop of plain dbscan2d calls over the same resident
+    clouds; labels must be identical."""
+    import numpy as np
+    import torch
+
+    from tests import workloads as W
+
+    ns, eps, ms = W.clouds2048_params()
+    t0 = time.perf_counter()
+    clouds = [W.clouds2048_cloud(i, int(ns[i])) for i in range(len(ns))]
+    gen_s = time.perf_counter() - t0
+    sizes = [len(c) for c in clouds]
+    offs = [0]
+    for k in sizes:
+        offs.append(offs[-1] + k)
+    packed = torch.from_numpy(np.concatenate(clouds)).cuda()
+    del clouds
+    e = [float(x) for x in eps]
+    m = [int(x) for x in ms]
+    B, M = len(sizes), offs[-1]
+    print(f"[bench] batched: {B} clouds, {M} points (generated in {gen_s:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    for _ in range(warmup):
+        dbscan_torch.dbscan2d_batched(packed, e, m, offsets=offs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    labels, kernel_ms, names, _ = dbscan_torch._C.dbscan2d_batched_benchmark(
+        packed, offs, e, m, steps)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+
+    views = [packed[offs[i]:offs[i + 1]] for i in range(B)]
+    for _ in range(max(1, warmup)):
+        for i, v in enumerate(views):
+            dbscan_torch.dbscan2d(v, e[i], m[i])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = [dbscan_torch.dbscan2d(v, e[i], m[i]) for i, v in enumerate(views)]
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    identical = bool(torch.equal(torch.cat(outs), labels))
+    return {
+        "workload": "clouds2048: 2,048 clouds, N_i log-uniform in [200, 500000], eps_i U[0.2,1.0], "
+                    "min_samples_i in {5,10,20}, master seed 4",
+        "clouds": B, "points": M, "steps": steps,
+        "ms_per_batch": round(t_batch * 1e3, 3),
+        "clouds_per_s": round(B / t_batch, 1),
+        "mpts_per_s": round(M / t_batch / 1e6, 1),
+        "looped_ms": round(t_loop * 1e3, 3),
+        "looped_clouds_per_s": round(B / t_loop, 1),
+        "speedup_vs_loop": round(t_loop / t_batch, 2),
+        "labels_identical": identical,
+        "kernels_ms": {nm: round(t, 4) for nm, t in zip(names, kernel_ms) if t > 0.0005},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +173,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="blobs10m", choices=["lattice1m", "blobs10m", "skewed10m"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 2,048-cloud batched-vs-looped leg")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-vs-looped leg (prints its JSON)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -135,6 +202,11 @@ def main() -> int:
 
     import dbscan_torch
     from tests import workloads as W
+
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(dbscan_torch, max(1, min(args.steps, 5)),
+                                                 min(args.warmup, 1))}), flush=True)
+        return 0
 
     X, eps, ms = W.WORKLOADS[args.workload]()
     n = len(X)
@@ -197,6 +269,10 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, eps, ms)
+    if not args.no_batched:
+        del Xd, labels
+        torch.cuda.empty_cache()
+        out["batched"] = batched_leg(dbscan_torch, 3, 1)
     print(json.dumps(out), flush=True)
     return 0
 

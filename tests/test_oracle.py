@@ -212,3 +212,18 @@ def test_planted_mistakes_are_caught():
         first[c] = ukey[lab == c].min()
     unsigned_order = sorted(first, key=first.get)
     assert unsigned_order != sorted(first)
+
+
+def test_fast_row_shuffle_is_generator_shuffle():
+    """The workloads gather rows by a shuffled index instead of shuffling the
+    2-D array in place; both consume the generator identically."""
+    from tests.workloads import _shuffle_rows
+
+    for n in (1, 2, 17, 5000):
+        X = np.random.default_rng(n).normal(size=(n, 2)).astype(np.float32)
+        a = X.copy()
+        r1 = np.random.default_rng(99)
+        r1.shuffle(a)
+        r2 = np.random.default_rng(99)
+        assert np.array_equal(_shuffle_rows(r2, X), a)
+        assert r1.integers(1 << 30) == r2.integers(1 << 30)
