@@ -19,6 +19,9 @@ min_samples 10, seed 2).  Prints one JSON line:
   batched        2,048 clouds (~135M points, per-cloud eps and min_samples):
                  clouds/s of one batched call against a Python loop of plain
                  dbscan2d calls over the same resident clouds, labels compared
+  serving        open-loop Poisson request streams through the continuous-
+                 batching server: p50/p99 latency and clouds/s per rate, and
+                 serial plain calls on the same requests
 """
 
 from __future__ import annotations
@@ -166,6 +169,73 @@ def batched_leg(dbscan_torch, steps: int, warmup: int) -> dict:
     }
 
 
+def serving_leg(dbscan_torch, rates, count: int, pool: int = 192) -> dict:
+    """Open-loop Poisson request streams served by the continuous-batching
+    server (pooled arena, 3 slots of 32M points): p50 / p99 latency and
+    clouds/s per rate, against serial plain dbscan2d calls on the same
+    requests.  Request i of a stream draws eps_i and min_samples_i from the
+    stream and its points from a pool of `pool` pre-generated pinned clouds
+    whose sizes follow the N mixture (generating every request's points on
+    the host would take minutes)."""
+    import numpy as np
+    import torch
+
+    from dbscan_torch.serving import Request, Server, latency_summary, run_open_loop
+    from tests import workloads as W
+
+    _, pool_ns, _, _ = W.serve_stream(1000, pool, seed=6)
+    t0 = time.perf_counter()
+    clouds = [torch.from_numpy(W.cloud_at_density(int(k), 5000 + j)).pin_memory()
+              for j, k in enumerate(pool_ns)]
+    print(f"[bench] serving: pool of {pool} clouds, {int(pool_ns.sum())} points "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    server = Server(slots=3, slot_points=32 << 20, slot_clouds=4096)
+
+    def make(rate):
+        arr, _, eps, ms = W.serve_stream(rate, count)
+        reqs = [Request(i, clouds[i % pool], float(eps[i]), int(ms[i])) for i in range(count)]
+        return arr, reqs
+
+    # warm-up: one short stream
+    arr, reqs = make(rates[0])
+    run_open_loop(server, reqs[:100], arr[:100])
+    out = {"workload": "serve: open-loop Poisson, N mixture 70% log-U[1k,50k], 25% "
+                       "log-U[50k,1M], 5% log-U[1M,10M], eps U[0.2,1.0], min_samples {5,10,20}",
+           "requests_per_rate": count, "pool_clouds": pool,
+           "arena_gb": round(server.arena_bytes / 1e9, 2), "slots": len(server.slots),
+           "rates": {}}
+    for rate in rates:
+        arr, reqs = make(rate)
+        nb = len(server.batches)
+        t0 = time.perf_counter()
+        run_open_loop(server, reqs, arr)
+        wall = time.perf_counter() - t0
+        summ = latency_summary(reqs)
+        b = server.batches[nb:]
+        summ.update({"offered_per_s": rate, "wall_s": round(wall, 3), "batches": len(b),
+                     "mean_batch": round(sum(b) / max(1, len(b)), 2)})
+        out["rates"][str(rate)] = summ
+        print(f"[bench] serving rate {rate}: {summ}", file=sys.stderr, flush=True)
+
+    # serial plain calls on the first requests of the first stream
+    arr, reqs = make(rates[0])
+    k = min(count, 400)
+    for r in reqs[:3]:
+        dbscan_torch.dbscan2d(r.X.cuda(non_blocking=True), r.eps, r.min_samples).cpu()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = [dbscan_torch.dbscan2d(r.X.cuda(non_blocking=True), r.eps, r.min_samples).cpu()
+             for r in reqs[:k]]
+    serial = time.perf_counter() - t0
+    server_reqs = run_open_loop(server, reqs[:k], [0.0] * k)  # all at once: peak throughput
+    burst = max(r.done for r in server_reqs) - min(r.arrival for r in server_reqs)
+    out["serial_plain_clouds_per_s"] = round(k / serial, 1)
+    out["burst_served_clouds_per_s"] = round(k / burst, 1)
+    out["labels_identical_to_plain"] = all(torch.equal(p, r.labels) for p, r in
+                                           zip(plain, server_reqs))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +247,13 @@ def main() -> int:
                     help="skip the 2,048-cloud batched-vs-looped leg")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-vs-looped leg (prints its JSON)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving leg (prints its JSON)")
+    ap.add_argument("--serve-rates", default="500,1000,2000,5000",
+                    help="comma-separated request rates (req/s) of the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests per rate in the serving leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -203,6 +280,11 @@ def main() -> int:
     import dbscan_torch
     from tests import workloads as W
 
+    rates = [int(r) for r in args.serve_rates.split(",") if r]
+    if args.serving_only:
+        print(json.dumps({"serving": serving_leg(dbscan_torch, rates, args.serve_requests)}),
+              flush=True)
+        return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(dbscan_torch, max(1, min(args.steps, 5)),
                                                  min(args.warmup, 1))}), flush=True)
@@ -273,6 +355,9 @@ def main() -> int:
         del Xd, labels
         torch.cuda.empty_cache()
         out["batched"] = batched_leg(dbscan_torch, 3, 1)
+        torch.cuda.empty_cache()
+    if not args.no_serving:
+        out["serving"] = serving_leg(dbscan_torch, rates, args.serve_requests)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -218,6 +218,40 @@ torch::Tensor dbscan2d_batched(torch::Tensor X, std::vector<int64_t> offsets,
   return finish(X, labels);
 }
 
+// Serving helper (no reference counterpart): runs a batch into buffers the
+// caller owns -- X (N, 2) float32, labels (>= N,) int32 and a uint8 workspace,
+// all on one ROCm device, typically views into a pooled arena -- on the
+// current stream, without allocating.  Returns nothing; the work is enqueued.
+void dbscan2d_batched_out(torch::Tensor X, std::vector<int64_t> offsets, std::vector<double> eps,
+                          std::vector<int64_t> min_samples, torch::Tensor labels,
+                          torch::Tensor workspace) {
+  const Batch b = check_batch(X, offsets, eps, min_samples);
+  const int64_t n = X.size(0), B = static_cast<int64_t>(eps.size());
+  TORCH_CHECK(X.is_cuda() && labels.is_cuda() && workspace.is_cuda() &&
+                  labels.device() == X.device() && workspace.device() == X.device(),
+              "dbscan2d_batched_out: X, labels and workspace must be on one ROCm device");
+  TORCH_CHECK(X.is_contiguous() && labels.is_contiguous() && workspace.is_contiguous(),
+              "dbscan2d_batched_out: buffers must be contiguous");
+  TORCH_CHECK(labels.scalar_type() == torch::kInt32 && labels.numel() >= n,
+              "dbscan2d_batched_out: labels must be int32 with at least N elements");
+  TORCH_CHECK(workspace.scalar_type() == torch::kUInt8,
+              "dbscan2d_batched_out: workspace must be uint8");
+  if (n == 0) return;
+  c10::DeviceGuard guard(X.device());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  int bad = -1;
+  check_status(dbscan2d_hip_batched(X.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
+                                    b.ms.data(), B, labels.data_ptr<int32_t>(),
+                                    workspace.data_ptr(), static_cast<size_t>(workspace.numel()),
+                                    stream, &bad));
+}
+
+int64_t dbscan2d_batched_workspace_bytes(int64_t total_points, int64_t num_clouds) {
+  size_t bytes = 0;
+  check_status(dbscan2d_hip_batched_workspace_bytes(total_points, num_clouds, &bytes));
+  return static_cast<int64_t>(bytes);
+}
+
 namespace {
 
 // `steps` timed calls of `launch(events)`; returns per-kernel means and the
@@ -341,6 +375,14 @@ PYBIND11_MODULE(_C, m) {
         "Batched DBSCAN: cloud i = rows [offsets[i], offsets[i+1]) of the packed (N, 2) float32 "
         "X, clustered with eps[i], min_samples[i]. Returns the packed int32 labels, numbered "
         "from 0 within every cloud, on X's device.");
+  m.def("dbscan2d_batched_out", &dbscan2d_batched_out, pybind11::arg("X"),
+        pybind11::arg("offsets"), pybind11::arg("eps"), pybind11::arg("min_samples"),
+        pybind11::arg("labels"), pybind11::arg("workspace"),
+        "Serving helper: batched DBSCAN into caller-owned device buffers on the current "
+        "stream; allocates nothing.");
+  m.def("dbscan2d_batched_workspace_bytes", &dbscan2d_batched_workspace_bytes,
+        pybind11::arg("total_points"), pybind11::arg("num_clouds"),
+        "Workspace bytes of a batch of num_clouds clouds holding total_points points.");
   m.def("dbscan2d_batched_benchmark", &dbscan2d_batched_benchmark, pybind11::arg("X"),
         pybind11::arg("offsets"), pybind11::arg("eps"), pybind11::arg("min_samples"),
         pybind11::arg("steps"),

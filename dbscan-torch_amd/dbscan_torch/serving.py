@@ -1,0 +1,235 @@
+"""Continuous-batching request loop for dbscan2d on one GPU.
+
+No counterpart in the reference, which clusters one cloud per call.  Requests
+(a cloud of float32 (N, 2) points with its own eps and min_samples) arrive at
+any time; the server keeps a pooled HBM arena split into ``slots``, each able
+to hold one batch (points, labels and the batched pipeline's workspace).
+Whenever a slot is free, every queued request that fits is admitted into a
+batch by its HBM footprint (first come, first served), copied into the slot
+and clustered by one batched launch (dbscan2d_batched) on the slot's own
+stream, so up to ``slots`` batches are on the device at once: requests that
+arrive while others run are taken into the next batch as soon as a slot
+frees, and their copies and kernels overlap the batches in flight.  A
+request's labels are exactly what ``dbscan2d`` gives its cloud alone.
+
+Typical use::
+
+    server = Server(slots=3, slot_points=32 << 20)
+    server.submit(Request(0, X_host, eps=0.5, min_samples=10))
+    server.drain()          # or call server.step() from your own loop
+    labels = server.completed[0].labels
+"""
+
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Callable, Deque, Dict, List, Optional, Sequence
+
+import torch
+
+from . import _C
+
+_ALIGN = 256
+
+
+def _align(n: int) -> int:
+    return (n + _ALIGN - 1) // _ALIGN * _ALIGN
+
+
+@dataclasses.dataclass
+class Request:
+    """One dbscan2d call.  ``X`` is a float32 (N, 2) tensor, on the host
+    (pinned memory lets its copy overlap) or on the server's device.
+    ``arrival`` is the time the request is due (clock seconds); latency is
+    measured from it to the moment its labels are in host memory."""
+
+    rid: int
+    X: torch.Tensor
+    eps: float
+    min_samples: int
+    arrival: float = 0.0
+    labels: Optional[torch.Tensor] = None
+    admitted: Optional[float] = None
+    done: Optional[float] = None
+
+    @property
+    def n(self) -> int:
+        return int(self.X.shape[0])
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+class _Slot:
+    """One batch's share of the arena: points, labels, workspace, its stream
+    and a pinned host buffer for the labels."""
+
+    def __init__(self, arena: torch.Tensor, offset: int, points: int, ws_bytes: int,
+                 device: torch.device):
+        x_bytes, l_bytes = _align(points * 8), _align(points * 4)
+        self.X = arena.narrow(0, offset, points * 8).view(torch.float32).view(points, 2)
+        self.labels = arena.narrow(0, offset + x_bytes, points * 4).view(torch.int32)
+        self.workspace = arena.narrow(0, offset + x_bytes + l_bytes, ws_bytes)
+        self.stream = torch.cuda.Stream(device)  # default priority
+        self.host_labels = torch.empty(points, dtype=torch.int32, pin_memory=True)
+        self.event: Optional[torch.cuda.Event] = None
+        self.batch: List[Request] = []
+        self.offsets: List[int] = []
+
+    @staticmethod
+    def nbytes(points: int, ws_bytes: int) -> int:
+        return _align(points * 8) + _align(points * 4) + _align(ws_bytes)
+
+
+class Server:
+    """Pooled-arena, continuous-batching dbscan2d server on one GPU.
+
+    Args:
+        device: the ROCm device.
+        slots: batches that may be on the device at once.
+        slot_points: points one batch holds (the largest admissible request).
+        slot_clouds: requests one batch holds.
+        cap_bytes: the resident HBM budget; the arena (slots x slot bytes)
+            must fit in it.
+        clock: time source for admission and completion stamps.
+    """
+
+    def __init__(self, device="cuda", slots: int = 3, slot_points: int = 32 << 20,
+                 slot_clouds: int = 4096, cap_bytes: float = 200e9,
+                 clock: Callable[[], float] = time.perf_counter):
+        if slots < 1 or slot_points < 1 or slot_clouds < 1:
+            raise ValueError("Server: slots, slot_points and slot_clouds must be >= 1")
+        self.device = torch.device(device)
+        self.slot_points, self.slot_clouds, self.clock = slot_points, slot_clouds, clock
+        ws = _C.dbscan2d_batched_workspace_bytes(slot_points, slot_clouds)
+        slot_bytes = _Slot.nbytes(slot_points, ws)
+        self.arena_bytes = slots * slot_bytes
+        if self.arena_bytes > cap_bytes:
+            raise ValueError(f"Server: arena of {self.arena_bytes / 1e9:.1f} GB exceeds the "
+                             f"{cap_bytes / 1e9:.1f} GB resident cap")
+        self.arena = torch.empty(self.arena_bytes, dtype=torch.uint8, device=self.device)
+        self.slots = [_Slot(self.arena, i * slot_bytes, slot_points, ws, self.device)
+                      for i in range(slots)]
+        self.bytes_per_point = slot_bytes / slot_points
+        self.queue: Deque[Request] = collections.deque()
+        self.free: Deque[_Slot] = collections.deque(self.slots)
+        self.inflight: Deque[_Slot] = collections.deque()
+        self.completed: Dict[int, Request] = {}
+        self.batches: List[int] = []  # requests per launched batch
+
+    def footprint_bytes(self, n: int) -> int:
+        """Arena bytes a request of n points occupies (points, labels and its
+        share of the workspace)."""
+        return int(n * self.bytes_per_point)
+
+    def submit(self, req: Request) -> None:
+        X = req.X
+        if X.dim() != 2 or X.size(1) != 2:
+            raise RuntimeError("X must have shape (N, 2)")
+        if X.dtype != torch.float32:
+            raise RuntimeError("X must be float32")
+        if not req.eps > 0.0:
+            raise RuntimeError("eps must be > 0")
+        if req.min_samples < 1:
+            raise RuntimeError("min_samples must be >= 1")
+        if req.n > self.slot_points:
+            raise ValueError(f"Server: request {req.rid} has {req.n} points; a slot holds "
+                             f"{self.slot_points}")
+        if req.n == 0:
+            req.labels = torch.empty(0, dtype=torch.int32)
+            req.admitted = req.done = self.clock()
+            self.completed[req.rid] = req
+            return
+        self.queue.append(req)
+
+    def busy(self) -> bool:
+        return bool(self.queue) or bool(self.inflight)
+
+    def _launch(self, slot: _Slot) -> None:
+        batch, offs, pts = [], [0], 0
+        while (self.queue and len(batch) < self.slot_clouds
+               and pts + self.queue[0].n <= self.slot_points):
+            r = self.queue.popleft()
+            batch.append(r)
+            pts += r.n
+            offs.append(pts)
+        now = self.clock()
+        with torch.cuda.stream(slot.stream):
+            for r, o in zip(batch, offs):
+                r.admitted = now
+                slot.X[o:o + r.n].copy_(r.X, non_blocking=True)
+            _C.dbscan2d_batched_out(slot.X[:pts], offs, [r.eps for r in batch],
+                                    [r.min_samples for r in batch], slot.labels,
+                                    slot.workspace)
+            slot.host_labels[:pts].copy_(slot.labels[:pts], non_blocking=True)
+            slot.event = torch.cuda.Event()
+            slot.event.record(slot.stream)
+        slot.batch, slot.offsets = batch, offs
+        self.inflight.append(slot)
+        self.batches.append(len(batch))
+
+    def _retire(self, slot: _Slot) -> None:
+        now = self.clock()
+        for r, o in zip(slot.batch, slot.offsets):
+            r.labels = slot.host_labels[o:o + r.n].clone()
+            r.done = now
+            self.completed[r.rid] = r
+        slot.batch, slot.offsets, slot.event = [], [], None
+        self.free.append(slot)
+
+    def step(self) -> bool:
+        """Retires finished batches and launches new ones; never blocks.
+        Returns True if anything happened."""
+        progressed = False
+        for slot in list(self.inflight):
+            if slot.event.query():
+                self.inflight.remove(slot)
+                self._retire(slot)
+                progressed = True
+        while self.queue and self.free:
+            self._launch(self.free.popleft())
+            progressed = True
+        return progressed
+
+    def drain(self) -> None:
+        while self.busy():
+            if not self.step():
+                time.sleep(20e-6)
+
+
+def run_open_loop(server: Server, requests: Sequence[Request],
+                  arrivals: Sequence[float]) -> List[Request]:
+    """Serves ``requests`` as an open-loop stream: request i is submitted once
+    ``arrivals[i]`` seconds have passed since the start (never earlier), and
+    its latency runs from that scheduled time.  Returns the requests."""
+    order = sorted(range(len(requests)), key=lambda i: arrivals[i])
+    t0 = server.clock()
+    k = 0
+    while k < len(order) or server.busy():
+        now = server.clock() - t0
+        while k < len(order) and arrivals[order[k]] <= now:
+            r = requests[order[k]]
+            r.arrival = t0 + arrivals[order[k]]
+            server.submit(r)
+            k += 1
+        if not server.step() and (k == len(order) or arrivals[order[k]] > now + 50e-6):
+            time.sleep(20e-6)
+    return list(requests)
+
+
+def latency_summary(reqs: Sequence[Request]) -> dict:
+    """p50 / p99 / max latency in ms and completed clouds per second."""
+    lat = sorted(r.latency for r in reqs)
+    if not lat:
+        return {"requests": 0}
+
+    def pct(p):
+        return lat[min(len(lat) - 1, int(round(p / 100.0 * (len(lat) - 1))))] * 1e3
+
+    span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+    return {"requests": len(lat), "p50_ms": round(pct(50), 3), "p99_ms": round(pct(99), 3),
+            "max_ms": round(lat[-1] * 1e3, 3),
+            "clouds_per_s": round(len(lat) / span, 1) if span > 0 else None}

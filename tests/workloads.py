@@ -21,6 +21,9 @@ Named workloads:
          cloud i = G(N_i, max(1, round(8 N_i / 1e4)), [-L/2, L/2], 0.5, 0.05,
          [-0.6 L, 0.6 L], 1000 + i) with L = 20 sqrt(N_i / 1e4);
          eps_i = f32(U[0.2, 1.0]), min_samples_i from {5, 10, 20}
+  serve    open-loop Poisson stream per rate (seed [5, rate]): N_i from the
+         mixture 70% log-U[1k,50k], 25% log-U[50k,1M], 5% log-U[1M,10M],
+         eps_i and min_samples_i as clouds2048, clouds at blobs10k's density
 """
 
 from __future__ import annotations
@@ -93,9 +96,30 @@ def clouds2048_params(n_clouds: int = 2048, seed: int = 4):
 
 
 def clouds2048_cloud(i: int, n_i: int) -> np.ndarray:
-    L = 20.0 * np.sqrt(n_i / 1e4)
-    nb = max(1, int(round(8 * n_i / 1e4)))
-    return G(int(n_i), nb, (-L / 2, L / 2), 0.5, 0.05, (-0.6 * L, 0.6 * L), 1000 + i)
+    return cloud_at_density(n_i, 1000 + i)
+
+
+def cloud_at_density(n: int, seed: int) -> np.ndarray:
+    """A cloud of n points at blobs10k's density (the generator the batched
+    and served workloads draw from)."""
+    L = 20.0 * np.sqrt(n / 1e4)
+    nb = max(1, int(round(8 * n / 1e4)))
+    return G(int(n), nb, (-L / 2, L / 2), 0.5, 0.05, (-0.6 * L, 0.6 * L), seed)
+
+
+def serve_stream(rate: float, count: int, seed: int = 5):
+    """Open-loop Poisson request stream: arrival times (s), N_i from the
+    mixture 70% log-U[1k, 50k], 25% log-U[50k, 1M], 5% log-U[1M, 10M], eps_i
+    = f32(U[0.2, 1.0]), min_samples_i from {5, 10, 20}."""
+    rng = np.random.default_rng([seed, int(rate)])
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    u = rng.uniform(0, 1, count)
+    lo = np.where(u < 0.70, 1e3, np.where(u < 0.95, 5e4, 1e6))
+    hi = np.where(u < 0.70, 5e4, np.where(u < 0.95, 1e6, 1e7))
+    ns = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi)))).astype(np.int64)
+    eps = rng.uniform(0.2, 1.0, count).astype(np.float32)
+    ms = rng.choice(np.array([5, 10, 20]), count)
+    return arrivals, ns, eps, ms
 
 
 WORKLOADS = {"blobs10k": blobs10k, "lattice1m": lattice1m, "blobs10m": blobs10m, "skewed10m": skewed10m}
