@@ -17,7 +17,9 @@
 #include <c10/hip/HIPFunctions.h>
 #include <c10/hip/HIPStream.h>
 
+#include <algorithm>
 #include <map>
+#include <optional>
 #include <string>
 #include <tuple>
 #include <utility>
@@ -43,6 +45,23 @@ void check_status(int rc) {
   TORCH_CHECK(rc == DBSCAN_HIP_OK, "dbscan2d: ", dbscan2d_hip_status_string(rc));
 }
 
+// Launch configurations by input size, installed by the autotuner
+// (dbscan_torch.autotune); empty: the library's built-in table.
+std::vector<std::pair<int64_t, dbscan_hip_config>> g_tuning;
+
+const dbscan_hip_config* config_for(int64_t n) {
+  for (const auto& e : g_tuning)
+    if (n <= e.first) return &e.second;
+  return g_tuning.empty() ? nullptr : &g_tuning.back().second;
+}
+
+dbscan_hip_config to_config(const std::vector<int64_t>& v) {
+  TORCH_CHECK(v.size() == 4,
+              "config must be (sort_items, radix_bits, cell_blocks, pair_blocks)");
+  return dbscan_hip_config{static_cast<int32_t>(v[0]), static_cast<int32_t>(v[1]),
+                           static_cast<int32_t>(v[2]), static_cast<int32_t>(v[3])};
+}
+
 struct DeviceRun {
   torch::Tensor Xd;         // contiguous input on the HIP device
   torch::Tensor labels;     // int32 labels on the HIP device
@@ -50,7 +69,8 @@ struct DeviceRun {
 };
 
 // Runs the pipeline; stage_ms may be null.  Returns device tensors.
-DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, float* stage_ms) {
+DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, float* stage_ms,
+                     const dbscan_hip_config* cfg = nullptr) {
   c10::Device device = X.device();
   if (!X.is_cuda()) {
     TORCH_CHECK(c10::hip::device_count() > 0,
@@ -73,9 +93,9 @@ DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, fl
       stage_ms ? dbscan2d_hip_profile(r.Xd.data_ptr<float>(), n, eps, min_samples,
                                       r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(),
                                       ws_bytes, stream, stage_ms)
-               : dbscan2d_hip(r.Xd.data_ptr<float>(), n, eps, min_samples,
-                              r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(), ws_bytes,
-                              stream);
+               : dbscan2d_hip_configured(r.Xd.data_ptr<float>(), n, eps, min_samples,
+                                         r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(),
+                                         ws_bytes, stream, cfg ? cfg : config_for(n));
   check_status(rc);
   return r;
 }
@@ -93,6 +113,41 @@ torch::Tensor dbscan2d(torch::Tensor X, double eps, int64_t min_samples) {
     return torch::empty({0}, X.options().dtype(torch::kInt32));
   }
   return finish(X, run_device(X, eps, min_samples, nullptr).labels);
+}
+
+// dbscan2d with an explicit launch configuration (no reference counterpart;
+// for the autotuner and its parity tests).
+torch::Tensor dbscan2d_configured(torch::Tensor X, double eps, int64_t min_samples,
+                                  std::vector<int64_t> config) {
+  check_input(X, eps, min_samples);
+  if (X.size(0) == 0) return torch::empty({0}, X.options().dtype(torch::kInt32));
+  const dbscan_hip_config cfg = to_config(config);
+  return finish(X, run_device(X, eps, min_samples, nullptr, &cfg).labels);
+}
+
+void set_tuning_table(std::vector<std::vector<int64_t>> rows) {
+  std::vector<std::pair<int64_t, dbscan_hip_config>> t;
+  for (const auto& r : rows) {
+    TORCH_CHECK(r.size() == 5, "tuning rows are (max_n, sort_items, radix_bits, cell_blocks, "
+                               "pair_blocks)");
+    t.emplace_back(r[0], to_config({r[1], r[2], r[3], r[4]}));
+  }
+  std::sort(t.begin(), t.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  g_tuning = std::move(t);
+}
+
+std::vector<std::vector<int64_t>> get_tuning_table() {
+  std::vector<std::vector<int64_t>> out;
+  for (const auto& e : g_tuning)
+    out.push_back({e.first, e.second.sort_items, e.second.radix_bits, e.second.cell_blocks,
+                   e.second.pair_blocks});
+  return out;
+}
+
+std::vector<int64_t> default_config(int64_t n) {
+  dbscan_hip_config c;
+  check_status(dbscan2d_hip_default_config(n, &c));
+  return {c.sort_items, c.radix_bits, c.cell_blocks, c.pair_blocks};
 }
 
 // csrc/extension.cpp:52-63: labels plus GPU milliseconds per reference stage.
@@ -214,7 +269,8 @@ torch::Tensor dbscan2d_batched(torch::Tensor X, std::vector<int64_t> offsets,
   int bad = -1;
   check_status(dbscan2d_hip_batched(Xd.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
                                     b.ms.data(), B, labels.data_ptr<int32_t>(),
-                                    workspace.data_ptr(), ws_bytes, stream, &bad));
+                                    workspace.data_ptr(), ws_bytes, stream, &bad,
+                                    config_for(n)));
   return finish(X, labels);
 }
 
@@ -243,7 +299,7 @@ void dbscan2d_batched_out(torch::Tensor X, std::vector<int64_t> offsets, std::ve
   check_status(dbscan2d_hip_batched(X.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
                                     b.ms.data(), B, labels.data_ptr<int32_t>(),
                                     workspace.data_ptr(), static_cast<size_t>(workspace.numel()),
-                                    stream, &bad));
+                                    stream, &bad, config_for(n)));
 }
 
 int64_t dbscan2d_batched_workspace_bytes(int64_t total_points, int64_t num_clouds) {
@@ -306,7 +362,8 @@ std::vector<std::string> kernel_names() {
 // kernel.  Returns (labels, per-kernel mean ms over the steps, kernel names,
 // event-measured ms from the first kernel of step 0 to the end of the last).
 std::tuple<torch::Tensor, std::vector<double>, std::vector<std::string>, double>
-dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t steps) {
+dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t steps,
+                   std::optional<std::vector<int64_t>> config) {
   check_input(X, eps, min_samples);
   TORCH_CHECK(X.is_cuda() && X.size(0) > 0 && steps >= 1,
               "dbscan2d_benchmark: needs a non-empty ROCm tensor and steps >= 1");
@@ -319,10 +376,16 @@ dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t ste
   auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
   auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
   hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  dbscan_hip_config cfg{};
+  const dbscan_hip_config* cp = config_for(n);
+  if (config) {
+    cfg = to_config(*config);
+    cp = &cfg;
+  }
   auto r = time_kernels(steps, [&](hipEvent_t* ev) {
     return dbscan2d_hip_timed(Xd.data_ptr<float>(), n, eps, min_samples,
                               labels.data_ptr<int32_t>(), workspace.data_ptr(), ws_bytes, stream,
-                              ev);
+                              ev, cp);
   });
   return {labels, r.first, kernel_names(), r.second};
 }
@@ -331,7 +394,8 @@ dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t ste
 // dbscan2d_benchmark: `steps` back-to-back batched calls on resident input.
 std::tuple<torch::Tensor, std::vector<double>, std::vector<std::string>, double>
 dbscan2d_batched_benchmark(torch::Tensor X, std::vector<int64_t> offsets, std::vector<double> eps,
-                           std::vector<int64_t> min_samples, int64_t steps) {
+                           std::vector<int64_t> min_samples, int64_t steps,
+                           std::optional<std::vector<int64_t>> config) {
   const Batch b = check_batch(X, offsets, eps, min_samples);
   TORCH_CHECK(X.is_cuda() && X.size(0) > 0 && steps >= 1,
               "dbscan2d_batched_benchmark: needs a non-empty ROCm tensor and steps >= 1");
@@ -344,10 +408,16 @@ dbscan2d_batched_benchmark(torch::Tensor X, std::vector<int64_t> offsets, std::v
   auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
   auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
   hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  dbscan_hip_config cfg{};
+  const dbscan_hip_config* cp = config_for(n);
+  if (config) {
+    cfg = to_config(*config);
+    cp = &cfg;
+  }
   auto r = time_kernels(steps, [&](hipEvent_t* ev) {
     return dbscan2d_hip_batched_timed(Xd.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
                                       b.ms.data(), B, labels.data_ptr<int32_t>(),
-                                      workspace.data_ptr(), ws_bytes, stream, ev);
+                                      workspace.data_ptr(), ws_bytes, stream, ev, cp);
   });
   return {labels, r.first, kernel_names(), r.second};
 }
@@ -366,8 +436,19 @@ PYBIND11_MODULE(_C, m) {
         pybind11::arg("min_samples"),
         "Debug variant: returns [labels, info, perm, xs, ys, point_cell, is_core, cell_start, "
         "cell_key, cell_flags, cell_ref, uf, cell_cluster]. Not part of the public API.");
+  m.def("dbscan2d_configured", &dbscan2d_configured, pybind11::arg("X"), pybind11::arg("eps"),
+        pybind11::arg("min_samples"), pybind11::arg("config"),
+        "dbscan2d with an explicit launch configuration (sort_items, radix_bits, cell_blocks, "
+        "pair_blocks).");
+  m.def("set_tuning_table", &set_tuning_table, pybind11::arg("rows"),
+        "Installs launch configurations by size: rows of (max_n, sort_items, radix_bits, "
+        "cell_blocks, pair_blocks); an empty list restores the built-in table.");
+  m.def("get_tuning_table", &get_tuning_table, "The installed tuning rows.");
+  m.def("default_config", &default_config, pybind11::arg("n"),
+        "The library's built-in configuration for n points.");
   m.def("dbscan2d_benchmark", &dbscan2d_benchmark, pybind11::arg("X"), pybind11::arg("eps"),
         pybind11::arg("min_samples"), pybind11::arg("steps"),
+        pybind11::arg("config") = pybind11::none(),
         "Benchmark helper: (labels, per-kernel mean ms, kernel names, total event ms) over "
         "`steps` back-to-back calls. Not part of the public API.");
   m.def("dbscan2d_batched", &dbscan2d_batched, pybind11::arg("X"), pybind11::arg("offsets"),
@@ -385,6 +466,6 @@ PYBIND11_MODULE(_C, m) {
         "Workspace bytes of a batch of num_clouds clouds holding total_points points.");
   m.def("dbscan2d_batched_benchmark", &dbscan2d_batched_benchmark, pybind11::arg("X"),
         pybind11::arg("offsets"), pybind11::arg("eps"), pybind11::arg("min_samples"),
-        pybind11::arg("steps"),
+        pybind11::arg("steps"), pybind11::arg("config") = pybind11::none(),
         "Benchmark helper for batches, as dbscan2d_benchmark. Not part of the public API.");
 }

@@ -49,17 +49,20 @@
 namespace {
 
 constexpr int32_t kNoise = -1;
-// Radix digits are at most kRadixBitsMax wide.  A key of b bits is sorted in
-// P = ceil(b / kRadixBitsMax) passes of ceil(b / P) bits each, both chosen on
-// the device once the bounds are known (26-bit keys: 4 passes of 7 bits).  In
-// a batch every cloud takes the batch's largest P and splits its own key bits
-// over those P passes.
+// Radix digits are at most r bits wide (r = the configuration's radix_bits,
+// at most kRadixBitsMax).  A key of b bits is sorted in P = min(ceil(b / r),
+// kMaxPasses) passes of ceil(b / P) bits each, both chosen on the device once
+// the bounds are known (26-bit keys, r = 8: 4 passes of 7 bits).  In a batch
+// every cloud takes the batch's largest P and splits its own key bits over
+// those P passes.
 constexpr int kRadixBitsMax = 8;
 constexpr int kRadixMax = 1 << kRadixBitsMax;
 constexpr int kMaxPasses = 8;           // 64-bit keys at most
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 12;
-constexpr int kSortTile = kSortThreads * kSortItems;
+// A radix pass ranks tiles of kSortThreads * items points, items being one of
+// 8, 12, 16 (the configuration's sort_items); status words are sized for the
+// smallest tile.
+constexpr int kSortTileMin = kSortThreads * 8;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
@@ -122,22 +125,27 @@ __device__ __forceinline__ uint32_t bit_width(uint32_t v) {
   return v ? 32u - static_cast<uint32_t>(__clz(static_cast<int>(v))) : 0u;
 }
 
-// passes > 0: split the key over that many passes (a batch's common count).
+// passes > 0: split the key over that many passes (the count chosen when the
+// bounds were reduced, or a batch's common count).  Otherwise the passes are
+// as few as digits of at most `rbits` bits allow, and never more than
+// kMaxPasses (a wider key then takes wider digits, up to kRadixBitsMax).
 __device__ __forceinline__ KeyMap make_keymap(int32_t kxmin, int32_t kxmax, int32_t kymin,
-                                              int32_t kymax, int passes = 0) {
+                                              int32_t kymax, int passes = 0,
+                                              int rbits = kRadixBitsMax) {
   KeyMap m;
   m.kxmin = static_cast<uint32_t>(kxmin);
   m.kymin = static_cast<uint32_t>(kymin);
   m.bx = bit_width(static_cast<uint32_t>(kxmax) - m.kxmin);
   m.by = bit_width(static_cast<uint32_t>(kymax) - m.kymin);
   const int nbits = static_cast<int>(m.bx + m.by);
-  m.npasses = passes > 0 ? passes : (nbits ? (nbits + kRadixBitsMax - 1) / kRadixBitsMax : 1);
+  m.npasses = passes > 0 ? passes
+                         : (nbits ? min((nbits + rbits - 1) / rbits, kMaxPasses) : 1);
   m.dbits = nbits ? static_cast<uint32_t>((nbits + m.npasses - 1) / m.npasses) : 1u;
   return m;
 }
 
 __device__ __forceinline__ KeyMap load_keymap(const Params* p) {
-  return make_keymap(p->kxmin, p->kxmax, p->kymin, p->kymax);
+  return make_keymap(p->kxmin, p->kxmax, p->kymin, p->kymax, p->npasses);
 }
 
 // One cloud of a batch (dbscan2d_hip_batched): its point range in the packed
@@ -161,8 +169,9 @@ struct SegTile {
   int32_t cloud, start, count, first;
 };
 
-__device__ __forceinline__ KeyMap cloud_keymap(const CloudInfo& ci, int passes) {
-  return make_keymap(ci.kxmin, ci.kxmax, ci.kymin, ci.kymax, passes);
+__device__ __forceinline__ KeyMap cloud_keymap(const CloudInfo& ci, int passes,
+                                               int rbits = kRadixBitsMax) {
+  return make_keymap(ci.kxmin, ci.kxmax, ci.kymin, ci.kymax, passes, rbits);
 }
 
 // Scalar parameters of the cloud a cell belongs to.  The per-cell kernels
@@ -402,7 +411,8 @@ __global__ __launch_bounds__(256) void k_bounds(const float2* __restrict__ X, in
 __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int64_t n,
                                               float inv_r, const int4* __restrict__ partials,
                                               int nparts, Params* __restrict__ prm,
-                                              int2* __restrict__ col_range, int64_t max_cols) {
+                                              int2* __restrict__ col_range, int64_t max_cols,
+                                              int rbits) {
   __shared__ int32_t s_red[4][4];
   __shared__ uint32_t s_hist[kMaxPasses][kRadixMax];
   int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
     v[2] = min(v[2], q.z); v[3] = max(v[3], q.w);
   }
   block_minmax4(v, s_red);
-  const KeyMap m = make_keymap(v[0], v[1], v[2], v[3]);
+  const KeyMap m = make_keymap(v[0], v[1], v[2], v[3], 0, rbits);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     prm->kxmin = v[0]; prm->kxmax = v[1]; prm->kymin = v[2]; prm->kymax = v[3];
     prm->npasses = m.npasses;
@@ -491,7 +501,8 @@ __global__ __launch_bounds__(256) void k_bounds_seg(const float2* __restrict__ X
 // cloud whose columns would pass max_cols searches instead).
 __global__ __launch_bounds__(1024) void k_keys_seg(Params* __restrict__ prm,
                                                    CloudInfo* __restrict__ clouds,
-                                                   int32_t num_clouds, int64_t max_cols) {
+                                                   int32_t num_clouds, int64_t max_cols,
+                                                   int rbits) {
   __shared__ int32_t s_max[16];
   __shared__ uint32_t s_scan[16];
   __shared__ int64_t s_run;
@@ -499,7 +510,7 @@ __global__ __launch_bounds__(1024) void k_keys_seg(Params* __restrict__ prm,
   int32_t pmax = 1;
   for (int32_t i = t; i < num_clouds; i += 1024) {
     const CloudInfo& ci = clouds[i];
-    if (ci.end > ci.begin) pmax = max(pmax, cloud_keymap(ci, 0).npasses);
+    if (ci.end > ci.begin) pmax = max(pmax, cloud_keymap(ci, 0, rbits).npasses);
   }
   for (int d = 32; d >= 1; d >>= 1) pmax = max(pmax, __shfl_xor(pmax, d, 64));
   if (lane == 0) s_max[w] = pmax;
@@ -637,7 +648,7 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 
 // One stable LSD radix pass over digit `pass` of the compressed cell key.
 // kFirst: read the interleaved input X and emit original indices.
-// A tile of kSortTile points is ranked in LDS (wave64 ballot matching into
+// A tile of kSortThreads * kItems points is ranked in LDS (wave64 ballot matching into
 // per-wave counters, item by item, which keeps equal digits in input order),
 // offset by the global digit histogram and a look-back over earlier tiles,
 // then reordered by digit through one LDS buffer, array by array, so that runs
@@ -647,13 +658,14 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 // cloud's points; the cloud's own histogram, key map and scale apply, the
 // look-back stops at the cloud's first tile, and the cloud's points stay in
 // its own range [begin, end), so every cloud is sorted on its own.
-template <bool kFirst, bool kSeg>
+template <bool kFirst, bool kSeg, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
     int pass, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
     const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist) {
+  constexpr int kTile = kSortThreads * kItems;
   const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31: 32-bit offsets
   const uint32_t tile = blockIdx.x;  // see kTileOrder
   KeyMap m;
@@ -671,15 +683,15 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     hist = cloud_hist + (static_cast<size_t>(st.cloud) * kMaxPasses + pass) * kRadixMax;
   } else {
     m = load_keymap(prm);
-    base = tile * kSortTile;
-    tile_n = min(static_cast<uint32_t>(kSortTile), n - base);
+    base = tile * kTile;
+    tile_n = min(static_cast<uint32_t>(kTile), n - base);
     hist = prm->hist[pass];
   }
   if (pass >= m.npasses) return;
   constexpr int D = kDigitsPerThread;
 
-  __shared__ uint32_t s_buf[kSortTile];
-  __shared__ uint16_t s_d[kSortTile];
+  __shared__ uint32_t s_buf[kTile];
+  __shared__ uint16_t s_d[kTile];
   __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadixMax];
   __shared__ uint32_t s_goff[kRadixMax];
   __shared__ uint32_t s_dstart[kRadixMax];
@@ -705,12 +717,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     }
   }
   // local index of item k of this lane: wl + k * 64
-  const uint32_t wl = static_cast<uint32_t>(wave) * (64 * kSortItems) + lane;
+  const uint32_t wl = static_cast<uint32_t>(wave) * (64 * kItems) + lane;
 
-  float x[kSortItems], y[kSortItems];
-  uint32_t id[kSortItems], dg[kSortItems], rk[kSortItems];
+  float x[kItems], y[kItems];
+  uint32_t id[kItems], dg[kItems], rk[kItems];
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < kItems; ++k) {
     // unconditional loads (a clamped index past the end) keep the per-item
     // values in plain registers; validity is tested where it matters
     const uint32_t l = wl + k * 64;
@@ -726,14 +738,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   // all loads of the tile are in flight before the first key is computed
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) dg[k] = key_digit(sort_key(x[k], y[k], inv_r, m), pass, m);
+  for (int k = 0; k < kItems; ++k) dg[k] = key_digit(sort_key(x[k], y[k], inv_r, m), pass, m);
 
   // Rank within the wave's segment, item by item (stable): lanes holding the
   // same digit are found with one ballot per digit bit; the lowest such lane
   // bumps the wave's counter for that digit.
   const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < kItems; ++k) {
     const bool valid = wl + k * 64 < tile_n;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -789,7 +801,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   // Tile-local position of every item (kept in rk), then the digits in
   // position order.
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < kItems; ++k) {
     rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
     if (wl + k * 64 < tile_n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
   }
@@ -797,7 +809,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 #pragma unroll
   for (int arr = 0; arr < 3; ++arr) {
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < kItems; ++k) {
       if (wl + k * 64 < tile_n)
         s_buf[rk[k]] = arr == 0 ? __float_as_uint(x[k]) : (arr == 1 ? __float_as_uint(y[k]) : id[k]);
     }
@@ -1935,7 +1947,7 @@ constexpr size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(25
 Layout make_layout(int64_t n, int64_t clouds = 0) {
   const size_t N = static_cast<size_t>(n > 0 ? n : 1);
   const size_t B = static_cast<size_t>(clouds > 0 ? clouds : 0);
-  const size_t sort_tiles = (N + kSortTile - 1) / kSortTile + B;
+  const size_t sort_tiles = (N + kSortTileMin - 1) / kSortTileMin + B;
   const size_t scan_tiles = (N + kScanTile - 1) / kScanTile + B;
   Layout L{};
   size_t off = 0;
@@ -2030,9 +2042,35 @@ constexpr int kStageFirstKernel[DBSCAN_HIP_NUM_STAGES + 1] = {
     kKBounds, kKSort0, kKCells, kKNgh, kKCore, kKUnion, kKRank, kKLabel, kKBorder,
     kNumKernels, kNumKernels};
 
+// Built-in launch configuration by input size: the autotuner's table for
+// MI355X (dbscan_torch/tuning_gfx950.json); a caller can pass its own.
+dbscan_hip_config default_config(int64_t n) {
+  if (n <= 31622) return dbscan_hip_config{8, 8, 1024, 4096};
+  if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 2048};
+  if (n <= 3162277) return dbscan_hip_config{8, 8, 16384, 16384};
+  return dbscan_hip_config{12, 8, 4096, 8192};
+}
+
+bool config_ok(const dbscan_hip_config& c) {
+  return (c.sort_items == 8 || c.sort_items == 12 || c.sort_items == 16) && c.radix_bits >= 4 &&
+         c.radix_bits <= kRadixBitsMax && c.cell_blocks >= 64 && c.cell_blocks <= 65536 &&
+         c.pair_blocks >= 64 && c.pair_blocks <= 65536;
+}
+
+// A radix pass with `items` points per thread.
+template <bool kFirst, bool kSeg, class... A>
+void launch_sort(int items, int grid, hipStream_t st, A... a) {
+  if (items == 8)
+    k_sort_pass<kFirst, kSeg, 8><<<grid, kSortThreads, 0, st>>>(a...);
+  else if (items == 16)
+    k_sort_pass<kFirst, kSeg, 16><<<grid, kSortThreads, 0, st>>>(a...);
+  else
+    k_sort_pass<kFirst, kSeg, 12><<<grid, kSortThreads, 0, st>>>(a...);
+}
+
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
         void* workspace, size_t workspace_bytes, hipStream_t stream, float* stage_ms,
-        hipEvent_t* kev) {
+        hipEvent_t* kev, const dbscan_hip_config* cfgp) {
   if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
   if (!(eps > 0.0)) return DBSCAN_HIP_ERR_EPS;  // csrc/extension.cpp:29
   if (min_samples64 < 1) return DBSCAN_HIP_ERR_MIN_SAMPLES;  // csrc/extension.cpp:30
@@ -2045,6 +2083,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
     return DBSCAN_HIP_ERR_ALIGN;
   const Layout L = make_layout(n);
   if (workspace_bytes < L.total) return DBSCAN_HIP_ERR_WORKSPACE;
+  const dbscan_hip_config cfg = cfgp ? *cfgp : default_config(n);
+  if (!config_ok(cfg)) return DBSCAN_HIP_ERR_CONFIG;
 
   // Scalar parameters exactly as the reference computes them
   // (dbscan_cpu.cpp:52-53, dbscan_grid.hpp:153-154,425).
@@ -2118,12 +2158,12 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   const int g_reduce = grid_for(n, 2048, kMaxReduceBlocks);
   const int g_hist = grid_for(n, 4096, kMaxReduceBlocks);
-  const int g_sort = grid_for(n, kSortTile, INT_MAX);
+  const int g_sort = grid_for(n, kSortThreads * cfg.sort_items, INT_MAX);
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
   const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
-  const int g_cellwaves = grid_for(n, 256, 8192);  // one lane per cell, grid-stride
-  const int g_union = grid_for(n * 7, 256, 16384);
+  const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);  // one lane per cell, grid-stride
+  const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
 
   // Stage 1: cell keys (bounds + digit histograms).
   if ((e = before(kKBounds)) != hipSuccess) return fail(e);
@@ -2132,20 +2172,20 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
                                          at<uint4>(ws, kParamsZeroBegin), zero16);
   if ((e = before(kKHist)) != hipSuccess) return fail(e);
   k_hist<<<g_hist, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm,
-                                     at<int2>(ws, L.col_range), L.max_cols);
+                                     at<int2>(ws, L.col_range), L.max_cols, cfg.radix_bits);
 
   // Stage 2: radix passes (sort + gather in one).
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  k_sort_pass<true, false><<<g_sort, kSortThreads, 0, stream>>>(
-      X2, nullptr, nullptr, nullptr, xs_a, ys_a, idx_a, prm, sort_status, 0, n, inv_r, nullptr,
-      nullptr, nullptr);
+  launch_sort<true, false>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
+                           ys_a, idx_a, prm, sort_status, 0, n, inv_r, nullptr, nullptr,
+                           nullptr);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    k_sort_pass<false, false><<<g_sort, kSortThreads, 0, stream>>>(
-        nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
-        from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, sort_status, p,
-        n, inv_r, nullptr, nullptr, nullptr);
+    launch_sort<false, false>(
+        cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
+        from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
+        from_a ? idx_b : idx_a, prm, sort_status, p, n, inv_r, nullptr, nullptr, nullptr);
   }
 
   // Stage 3: cell runs.
@@ -2226,8 +2266,11 @@ bool cloud_params(double eps, int64_t min_samples64, CloudInfo& ci) {
 int run_batched(const float* X, int64_t total, const int64_t* offsets, const double* eps,
                 const int64_t* min_samples, int64_t num_clouds, int32_t* labels,
                 void* workspace, size_t workspace_bytes, hipStream_t stream, hipEvent_t* kev,
-                int* bad_cloud) {
+                int* bad_cloud, const dbscan_hip_config* cfgp) {
   if (bad_cloud) *bad_cloud = -1;
+  const dbscan_hip_config cfg = cfgp ? *cfgp : default_config(total);
+  if (!config_ok(cfg)) return DBSCAN_HIP_ERR_CONFIG;
+  const int32_t sort_tile = kSortThreads * cfg.sort_items;
   if (num_clouds < 0 || num_clouds > (1 << 24)) return DBSCAN_HIP_ERR_N;
   if (total < 0 || total >= INT32_MAX) return DBSCAN_HIP_ERR_N;
   if (num_clouds == 0) return total == 0 ? DBSCAN_HIP_OK : DBSCAN_HIP_ERR_N;
@@ -2252,7 +2295,7 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
     ci.kxmin = INT_MAX; ci.kxmax = INT_MIN; ci.kymin = INT_MAX; ci.kymax = INT_MIN;
     ci.col_base = -1;
     const int64_t ni = offsets[i + 1] - offsets[i];
-    sort_tiles += (ni + kSortTile - 1) / kSortTile;
+    sort_tiles += (ni + sort_tile - 1) / sort_tile;
     scan_tiles += (ni + kScanTile - 1) / kScanTile;
   }
   if (total == 0) return DBSCAN_HIP_OK;
@@ -2273,8 +2316,8 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
     int32_t a = 0, b = 0;
     for (int32_t i = 0; i < B; ++i) {
       const int32_t first = a;
-      for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kSortTile)
-        st_sort[a++] = SegTile{i, p0, std::min(kSortTile, info[i].end - p0), first};
+      for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += sort_tile)
+        st_sort[a++] = SegTile{i, p0, std::min(sort_tile, info[i].end - p0), first};
       for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kScanTile)
         st_scan[b++] = SegTile{i, p0, std::min(kScanTile, info[i].end - p0), 0};
     }
@@ -2327,8 +2370,8 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const int g_scan_cells = grid_for(n, kScanTile, INT_MAX);
   const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
-  const int g_cellwaves = grid_for(n, 256, 8192);
-  const int g_union = grid_for(n * 7, 256, 16384);
+  const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);
+  const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
   const int hist_per_block = static_cast<int>((sort_tiles + 1023) / 1024);
   const int g_hist = static_cast<int>((sort_tiles + hist_per_block - 1) / hist_per_block);
 
@@ -2336,21 +2379,21 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const int64_t zero16 = static_cast<int64_t>((L.zero_end - kParamsZeroBegin) / 16);
   k_bounds_seg<<<g_sort, 256, 0, stream>>>(X2, seg_sort, clouds, at<uint4>(ws, kParamsZeroBegin),
                                            zero16);
-  k_keys_seg<<<1, 1024, 0, stream>>>(prm, clouds, B, L.max_cols);
+  k_keys_seg<<<1, 1024, 0, stream>>>(prm, clouds, B, L.max_cols, cfg.radix_bits);
   if ((e = before(kKHist)) != hipSuccess) return fail(e);
   k_hist_seg<<<g_hist, 256, 0, stream>>>(X2, seg_sort, g_sort, hist_per_block, clouds, prm,
                                          cloud_hist, col_range);
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  k_sort_pass<true, true><<<g_sort, kSortThreads, 0, stream>>>(
-      X2, nullptr, nullptr, nullptr, xs_a, ys_a, idx_a, prm, sort_status, 0, n, 0.f, seg_sort,
-      clouds, cloud_hist);
+  launch_sort<true, true>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
+                          ys_a, idx_a, prm, sort_status, 0, n, 0.f, seg_sort, clouds,
+                          cloud_hist);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    k_sort_pass<false, true><<<g_sort, kSortThreads, 0, stream>>>(
-        nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
-        from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, sort_status, p,
-        n, 0.f, seg_sort, clouds, cloud_hist);
+    launch_sort<false, true>(
+        cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
+        from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
+        from_a ? idx_b : idx_a, prm, sort_status, p, n, 0.f, seg_sort, clouds, cloud_hist);
   }
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
   k_cells<true><<<g_scan_pts, kScanThreads, 0, stream>>>(
@@ -2407,15 +2450,28 @@ int dbscan2d_hip_workspace_bytes(int64_t n, size_t* bytes) {
 int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples, int32_t* labels,
                  void* workspace, size_t workspace_bytes, hipStream_t stream) {
   return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
-             nullptr);
+             nullptr, nullptr);
+}
+
+int dbscan2d_hip_default_config(int64_t n, dbscan_hip_config* cfg) {
+  if (!cfg) return DBSCAN_HIP_ERR_NULL;
+  *cfg = default_config(n);
+  return DBSCAN_HIP_OK;
+}
+
+int dbscan2d_hip_configured(const float* X, int64_t n, double eps, int64_t min_samples,
+                            int32_t* labels, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream, const dbscan_hip_config* cfg) {
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
+             nullptr, cfg);
 }
 
 int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_samples,
                        int32_t* labels, void* workspace, size_t workspace_bytes,
-                       hipStream_t stream, hipEvent_t* events) {
+                       hipStream_t stream, hipEvent_t* events, const dbscan_hip_config* cfg) {
   if (!events) return DBSCAN_HIP_ERR_NULL;
   return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
-             events);
+             events, cfg);
 }
 
 const char* dbscan2d_hip_kernel_name(int i) {
@@ -2427,7 +2483,7 @@ int dbscan2d_hip_profile(const float* X, int64_t n, double eps, int64_t min_samp
                          hipStream_t stream, float* stage_ms) {
   if (!stage_ms) return DBSCAN_HIP_ERR_NULL;
   return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, stage_ms,
-             nullptr);
+             nullptr, nullptr);
 }
 
 const char* dbscan2d_hip_stage_name(int i) {
@@ -2447,18 +2503,19 @@ int dbscan2d_hip_batched_workspace_bytes(int64_t total_points, int64_t num_cloud
 int dbscan2d_hip_batched(const float* X, int64_t total_points, const int64_t* offsets,
                          const double* eps, const int64_t* min_samples, int64_t num_clouds,
                          int32_t* labels, void* workspace, size_t workspace_bytes,
-                         hipStream_t stream, int* bad_cloud) {
+                         hipStream_t stream, int* bad_cloud, const dbscan_hip_config* cfg) {
   return run_batched(X, total_points, offsets, eps, min_samples, num_clouds, labels, workspace,
-                     workspace_bytes, stream, nullptr, bad_cloud);
+                     workspace_bytes, stream, nullptr, bad_cloud, cfg);
 }
 
 int dbscan2d_hip_batched_timed(const float* X, int64_t total_points, const int64_t* offsets,
                                const double* eps, const int64_t* min_samples,
                                int64_t num_clouds, int32_t* labels, void* workspace,
-                               size_t workspace_bytes, hipStream_t stream, hipEvent_t* events) {
+                               size_t workspace_bytes, hipStream_t stream, hipEvent_t* events,
+                               const dbscan_hip_config* cfg) {
   if (!events) return DBSCAN_HIP_ERR_NULL;
   return run_batched(X, total_points, offsets, eps, min_samples, num_clouds, labels, workspace,
-                     workspace_bytes, stream, events, nullptr);
+                     workspace_bytes, stream, events, nullptr, cfg);
 }
 
 int dbscan2d_hip_layout(int64_t n, dbscan_hip_layout* out) {
@@ -2489,6 +2546,7 @@ const char* dbscan2d_hip_status_string(int status) {
     case DBSCAN_HIP_ERR_WORKSPACE: return "workspace too small";
     case DBSCAN_HIP_ERR_ALIGN: return "misaligned pointer (X 8 B, labels 4 B, workspace 256 B)";
     case DBSCAN_HIP_ERR_HIP: return "HIP runtime error";
+    case DBSCAN_HIP_ERR_CONFIG: return "invalid launch configuration";
     default: return "unknown status";
   }
 }

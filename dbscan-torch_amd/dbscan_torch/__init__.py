@@ -41,6 +41,21 @@ except ImportError as exc:  # the native library is the product: fail loudly
         "dbscan_torch: the native extension (_C, libdbscan_hip.so) is not built or does not "
         "load. Build it with: python dbscan-torch_amd/build.py") from exc
 
+# Launch configurations saved by the autotuner (python -m dbscan_torch.autotune)
+# beside this file; without one the library's built-in table applies.
+def _load_tuning() -> None:
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_gfx950.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            rows = json.load(f)["rows"]
+        _C.set_tuning_table([[int(r["max_n"])] + [int(v) for v in r["config"]] for r in rows])
+
+
+_load_tuning()
+
 ArrayLike2D = Union[np.ndarray, torch.Tensor]
 
 __all__ = ["dbscan2d", "dbscan2d_batched", "DBSCAN", "ArrayLike2D"]

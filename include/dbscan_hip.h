@@ -32,8 +32,27 @@ enum {
   DBSCAN_HIP_ERR_NULL = 4,           /* a required pointer is NULL                 */
   DBSCAN_HIP_ERR_WORKSPACE = 5,      /* workspace smaller than the required bytes  */
   DBSCAN_HIP_ERR_ALIGN = 6,          /* X, labels or workspace misaligned          */
-  DBSCAN_HIP_ERR_HIP = 7             /* a HIP runtime call failed (see last error) */
+  DBSCAN_HIP_ERR_HIP = 7,            /* a HIP runtime call failed (see last error) */
+  DBSCAN_HIP_ERR_CONFIG = 8          /* invalid dbscan_hip_config                  */
 };
+
+/*
+ * Launch configuration (no counterpart in the reference, whose launch shapes
+ * are fixed in csrc/cuda/dbscan_cuda.cu).  Every valid configuration gives
+ * the same labels; they differ in speed only.  The entry points without a
+ * configuration argument (or given NULL) use the built-in table for the
+ * input size (dbscan2d_hip_default_config), which the project's autotuner
+ * (dbscan_torch.autotune) produces and can override per process.
+ */
+typedef struct {
+  int32_t sort_items;   /* points per thread of a radix-sort tile: 8, 12 or 16   */
+  int32_t radix_bits;   /* widest radix digit used when it saves no pass: 4..8   */
+  int32_t cell_blocks;  /* workgroups (of 256) of the per-cell kernels: 64..65536 */
+  int32_t pair_blocks;  /* workgroups of the cell-pair union kernels: 64..65536  */
+} dbscan_hip_config;
+
+/* The built-in configuration for n points. */
+int dbscan2d_hip_default_config(int64_t n, dbscan_hip_config* cfg);
 
 /* Number of stages reported by dbscan2d_hip_profile, and their names.  The
  * names are the reference's kStageNames (csrc/cuda/dbscan_cuda.cu:513-524,
@@ -64,6 +83,14 @@ int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples,
                  hipStream_t stream);
 
 /*
+ * dbscan2d_hip with an explicit launch configuration (NULL: the built-in
+ * one).  No counterpart in the reference: the autotuner's entry point.
+ */
+int dbscan2d_hip_configured(const float* X, int64_t n, double eps, int64_t min_samples,
+                            int32_t* labels, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream, const dbscan_hip_config* cfg);
+
+/*
  * Same as dbscan2d_hip, and additionally writes the GPU time of each of the
  * DBSCAN_HIP_NUM_STAGES stages, in milliseconds, to stage_ms (host memory).
  * Synchronises on its own events before returning.
@@ -87,12 +114,13 @@ const char* dbscan2d_hip_kernel_name(int i);
  * Same as dbscan2d_hip, and records events[k] on `stream` before kernel slot
  * k and events[DBSCAN_HIP_NUM_KERNELS] after the last one (the caller creates
  * the DBSCAN_HIP_NUM_KERNELS + 1 events).  Does not synchronise, so calls can
- * be timed back to back.  No counterpart in the reference: per-kernel timing
- * for the benchmark's roofline figures.
+ * be timed back to back; cfg as for dbscan2d_hip_configured.  No
+ * counterpart in the reference: per-kernel timing for the benchmark's
+ * roofline figures and the autotuner.
  */
 int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_samples,
                        int32_t* labels, void* workspace, size_t workspace_bytes,
-                       hipStream_t stream, hipEvent_t* events);
+                       hipStream_t stream, hipEvent_t* events, const dbscan_hip_config* cfg);
 
 /*
  * Batched form: num_clouds independent clouds packed one after another in X
@@ -105,7 +133,8 @@ int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_sample
  * and min_samples are HOST arrays, validated per cloud as dbscan2d_hip
  * validates its arguments; on a per-cloud error *bad_cloud (if not NULL)
  * names the cloud.  total_points < INT32_MAX.  The per-cloud records and tile
- * tables are copied into the workspace with hipMemcpyAsync on `stream`.
+ * tables are copied into the workspace with hipMemcpyAsync on `stream`.  cfg
+ * as for dbscan2d_hip_configured (NULL: built-in, by total_points).
  *
  * No counterpart in the reference, whose binding clusters one cloud per call
  * (csrc/extension.cpp:33-45); this is the batched launch that replaces a
@@ -116,7 +145,7 @@ int dbscan2d_hip_batched_workspace_bytes(int64_t total_points, int64_t num_cloud
 int dbscan2d_hip_batched(const float* X, int64_t total_points, const int64_t* offsets,
                          const double* eps, const int64_t* min_samples, int64_t num_clouds,
                          int32_t* labels, void* workspace, size_t workspace_bytes,
-                         hipStream_t stream, int* bad_cloud);
+                         hipStream_t stream, int* bad_cloud, const dbscan_hip_config* cfg);
 
 /*
  * dbscan2d_hip_batched with the event slots of dbscan2d_hip_timed (slot 0
@@ -127,7 +156,7 @@ int dbscan2d_hip_batched_timed(const float* X, int64_t total_points, const int64
                                const double* eps, const int64_t* min_samples,
                                int64_t num_clouds, int32_t* labels, void* workspace,
                                size_t workspace_bytes, hipStream_t stream,
-                               hipEvent_t* events);
+                               hipEvent_t* events, const dbscan_hip_config* cfg);
 
 /*
  * Byte offsets, inside the workspace of a dbscan2d_hip call on n points, of

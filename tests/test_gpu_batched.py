@@ -176,7 +176,7 @@ def test_c_abi_batched_direct(dt):
     lib = ctypes.CDLL(os.path.join(os.path.dirname(dt.__file__), "libdbscan_hip.so"))
     P = ctypes.c_void_p
     lib.dbscan2d_hip_batched.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int64, P, P,
-                                         ctypes.c_size_t, P, ctypes.POINTER(ctypes.c_int)]
+                                         ctypes.c_size_t, P, ctypes.POINTER(ctypes.c_int), P]
     lib.dbscan2d_hip_batched_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int64,
                                                          ctypes.POINTER(ctypes.c_size_t)]
     clouds, eps, ms = _random_batch(4, 9)
@@ -197,7 +197,7 @@ def test_c_abi_batched_direct(dt):
         return lib.dbscan2d_hip_batched(Xd.data_ptr(), total, offs.ctypes.data,
                                         eps_arr.ctypes.data, m.ctypes.data, len(clouds),
                                         labels.data_ptr(), ws.data_ptr(), ws_bytes, stream,
-                                        ctypes.byref(bad))
+                                        ctypes.byref(bad), None)
 
     assert call(e, nbytes.value) == 0 and bad.value == -1
     torch.cuda.synchronize()

@@ -219,3 +219,55 @@ def test_eight_byte_aligned_input(dt):
     view = big[1:]
     assert view.is_contiguous() and view.data_ptr() % 16 == 8
     assert np.array_equal(dt.dbscan2d(view, eps, ms).cpu().numpy(), oracle.dbscan2d(X, eps, ms))
+
+
+_CONFIGS = [(i, b, cb, pb) for i in (8, 12, 16) for b in (6, 7, 8)
+            for cb, pb in ((64, 64), (65536, 65536))]
+
+
+@pytest.fixture(scope="module")
+def lattice_ref():
+    X, eps, ms = W.lattice1m()
+    return X, eps, ms, oracle.dbscan2d(X, eps, ms)
+
+
+@pytest.mark.parametrize("cfg", _CONFIGS, ids=[f"i{c[0]}b{c[1]}g{c[2]}" for c in _CONFIGS])
+def test_every_launch_configuration_gives_the_reference_labels(dt, lattice_ref, cfg):
+    X, eps, ms, ref = lattice_ref
+    print(f"[gpu] config {cfg}", flush=True)
+    got = dt._C.dbscan2d_configured(torch.from_numpy(X).to(DEV), eps, ms, list(cfg))
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
+def test_batched_launch_configurations(dt):
+    rng = np.random.default_rng(8)
+    clouds = [W.cloud_at_density(int(n), 300 + i) for i, n in
+              enumerate(rng.integers(1, 40_000, 24))]
+    eps = [float(np.float32(e)) for e in rng.uniform(0.2, 1.0, 24)]
+    ms = [int(m) for m in rng.choice([5, 10, 20], 24)]
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).tolist()
+    packed = torch.from_numpy(np.concatenate(clouds)).to(DEV)
+    refs = [oracle.dbscan2d(c, e, m) for c, e, m in zip(clouds, eps, ms)]
+    for cfg in ((8, 6, 64, 64), (16, 8, 65536, 65536), (12, 7, 1024, 4096)):
+        labels = dt._C.dbscan2d_batched_benchmark(packed, offs, eps, ms, 1, list(cfg))[0]
+        labels = labels.cpu().numpy()
+        for i, r in enumerate(refs):
+            assert np.array_equal(labels[offs[i]:offs[i + 1]], r), (cfg, i)
+
+
+def test_autotune_bucket_and_table(dt, tmp_path):
+    from dbscan_torch import autotune
+
+    saved = dt._C.get_tuning_table()
+    try:
+        table = autotune.autotune(buckets=(20_000, 200_000), steps=1,
+                                  save=str(tmp_path / "t.json"))
+        assert [r["max_n"] for r in table["rows"]] == [63245, 2**31 - 1]
+        for b in table["buckets"]:
+            assert b["ms"] <= b["default_ms"] and len(b["tried"]) > 10
+        assert autotune.load(str(tmp_path / "t.json"))
+        X, eps, ms = W.blobs10k()
+        got = dt.dbscan2d(torch.from_numpy(X).to(DEV), eps, ms).cpu().numpy()
+        assert np.array_equal(got, oracle.dbscan2d(X, eps, ms))
+    finally:
+        dt._C.set_tuning_table(saved)

@@ -152,7 +152,7 @@ def test_batched_c_abi_argument_checks():
     lib = ctypes.CDLL(LIB)
     P = ctypes.c_void_p
     lib.dbscan2d_hip_batched.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_int64, P, P,
-                                         ctypes.c_size_t, P, ctypes.POINTER(ctypes.c_int)]
+                                         ctypes.c_size_t, P, ctypes.POINTER(ctypes.c_int), P]
     lib.dbscan2d_hip_batched_workspace_bytes.argtypes = [ctypes.c_int64, ctypes.c_int64,
                                                          ctypes.POINTER(ctypes.c_size_t)]
     offs = np.array([0, 6, 6, 10], dtype=np.int64)
@@ -163,7 +163,7 @@ def test_batched_c_abi_argument_checks():
     def call(o, e, m, total=10, B=3, ws=0x3000, ws_bytes=1 << 30):
         return lib.dbscan2d_hip_batched(0x1000, total, o.ctypes.data, e.ctypes.data,
                                         m.ctypes.data, B, 0x2000, ws, ws_bytes, None,
-                                        ctypes.byref(bad))
+                                        ctypes.byref(bad), None)
 
     assert call(offs, eps, ms) == 3 and bad.value == 1          # min_samples of cloud 1
     ms[1] = 5
@@ -179,3 +179,49 @@ def test_batched_c_abi_argument_checks():
         assert lib.dbscan2d_hip_batched_workspace_bytes(total, B, ctypes.byref(b)) == 0
         sizes.append(b.value)
     assert sizes[0] <= sizes[1] < sizes[2] < sizes[3]
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [("sort_items", ctypes.c_int32), ("radix_bits", ctypes.c_int32),
+                ("cell_blocks", ctypes.c_int32), ("pair_blocks", ctypes.c_int32)]
+
+
+def test_launch_configuration_checks():
+    lib = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    lib.dbscan2d_hip_configured.argtypes = [P, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
+                                            P, P, ctypes.c_size_t, P, ctypes.POINTER(_Cfg)]
+    lib.dbscan2d_hip_default_config.argtypes = [ctypes.c_int64, ctypes.POINTER(_Cfg)]
+    d = _Cfg()
+    assert lib.dbscan2d_hip_default_config(10_000_000, ctypes.byref(d)) == 0
+    assert d.sort_items in (8, 12, 16) and 4 <= d.radix_bits <= 8
+    for bad in ((10, 8, 8192, 8192), (12, 9, 8192, 8192), (12, 3, 8192, 8192),
+                (12, 8, 10, 8192), (12, 8, 8192, 1 << 20)):
+        c = _Cfg(*bad)
+        # fake, aligned addresses: the configuration is refused before any device work
+        assert lib.dbscan2d_hip_configured(0x1000, 10, 1.0, 5, 0x2000, 0x3000, 1 << 30, None,
+                                           ctypes.byref(c)) == 8, bad
+
+
+def test_tuning_table_round_trip():
+    from dbscan_torch import _C, autotune
+
+    saved = _C.get_tuning_table()
+    try:
+        autotune.install_rows([{"max_n": 50_000, "config": [8, 7, 2048, 4096]},
+                               {"max_n": 2**31 - 1, "config": [16, 8, 8192, 16384]}])
+        assert _C.get_tuning_table() == [[50_000, 8, 7, 2048, 4096],
+                                         [2**31 - 1, 16, 8, 8192, 16384]]
+        with pytest.raises(RuntimeError, match="tuning rows"):
+            _C.set_tuning_table([[1, 2, 3]])
+    finally:
+        _C.set_tuning_table(saved)
+    assert len(_C.default_config(1000)) == 4
+
+
+def test_synthetic_tuning_cloud_is_seeded():
+    from dbscan_torch.autotune import synthetic_cloud
+
+    a, b = synthetic_cloud(5000, 3), synthetic_cloud(5000, 3)
+    assert a.shape == (5000, 2) and a.dtype == torch.float32 and torch.equal(a, b)
+    assert not torch.equal(a, synthetic_cloud(5000, 4))
