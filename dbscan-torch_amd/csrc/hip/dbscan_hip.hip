@@ -264,36 +264,53 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
   return off + inc - v;
 }
 
+__device__ __forceinline__ int lowest_lane(uint64_t m) {
+  return __ffsll(static_cast<unsigned long long>(m)) - 1;
+}
+
 // Decoupled look-back for a single running total per tile (tags: 1 = tile
-// aggregate, 2 = inclusive prefix).  Called by one thread; returns the
-// exclusive prefix of `tile`.
-__device__ uint64_t lookback_single(uint64_t* status, uint32_t tile, uint64_t aggregate,
-                                    int32_t* error) {
+// aggregate, 2 = inclusive prefix), by the 64 lanes of one wave: each round
+// trip reads the status words of the 64 preceding tiles at once (lane k reads
+// tile j - k), adds the aggregates up to the first inclusive prefix, and
+// stops there; a word not yet published stops the round short of it and is
+// waited for, within kSpinLimit.  Returns the exclusive prefix of `tile` (in
+// every lane) and publishes the tile's inclusive prefix.
+__device__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggregate,
+                                  int32_t* error) {
   constexpr uint64_t kAgg = 1ull << 48, kPre = 2ull << 48;
+  const int lane = threadIdx.x & 63;
   if (tile == 0) {
-    st_release(status, kPre | aggregate);
+    if (lane == 0) st_release(status, kPre | aggregate);
     return 0;
   }
-  st_release(status + tile, kAgg | aggregate);
-  uint64_t excl = 0;
+  if (lane == 0) st_release(status + tile, kAgg | aggregate);
+  uint32_t excl = 0;  // counts are below 2^31
   int64_t j = static_cast<int64_t>(tile) - 1;
   uint32_t spins = 0;
-  while (j >= 0) {
-    const uint64_t s = ld_acquire(status + j);
+  for (;;) {
+    const int64_t q = j - lane;
+    const uint64_t s = q >= 0 ? ld_acquire(status + q) : kPre;  // before tile 0: prefix 0
     const uint64_t tag = s >> 48;
-    if (tag == 0) {
+    const uint64_t pre = __ballot(tag == 2), miss = __ballot(tag == 0);
+    const int first_pre = pre ? lowest_lane(pre) : 64;
+    const int first_miss = miss ? lowest_lane(miss) : 64;
+    const bool done = first_pre < first_miss;  // a prefix before any gap
+    const int use = done ? first_pre + 1 : first_miss;
+    uint32_t v = lane < use ? static_cast<uint32_t>(s & kCountMask) : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    excl += v;
+    if (done) break;
+    j -= use;
+    if (use == 0) {
       if (++spins > kSpinLimit) {
-        atomicOr(error, static_cast<int32_t>(kErrScanSpin));
+        if (lane == 0) atomicOr(error, static_cast<int32_t>(kErrScanSpin));
         break;
       }
       __builtin_amdgcn_s_sleep(1);
-      continue;
     }
-    excl += s & kCountMask;
-    if (tag == 2) break;
-    --j;
   }
-  st_release(status + tile, kPre | (excl + aggregate));
+  if (lane == 0) st_release(status + tile, kPre | (static_cast<uint64_t>(excl) + aggregate));
   return excl;
 }
 
@@ -596,8 +613,11 @@ __global__ __launch_bounds__(256) void k_hist_seg(const float2* __restrict__ X,
   flush();
 }
 
+// Predecessor status words a radix-pass look-back reads per round trip.
+constexpr int kSortLookback = 4;
+
 // Sum of the earlier tiles' counts of one digit (decoupled look-back).  The
-// status words of four predecessors are read per round trip; a word tagged
+// status words of kSortLookback predecessors are read per round trip; a word tagged
 // `tag_pre` is an inclusive prefix (stop), `tag_agg` a tile's own count
 // (add, keep going), anything else is not yet published (or left by an
 // earlier pass) and is waited for, within kSpinLimit.
@@ -610,9 +630,9 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
   int64_t j = tile - 1;
   uint32_t spins = 0;
   while (j >= first) {
-    uint64_t w[4];
+    uint64_t w[kSortLookback];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < kSortLookback; ++k)
       w[k] = (j - k >= first)
                  ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadixMax + digit)
                  : tag_pre;
@@ -620,7 +640,7 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
     int used = 0;
     bool stop = false, blocked = false;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kSortLookback; ++k) {
       const uint64_t tag = w[k] & ~kCountMask;
       const bool ready = tag == tag_agg || tag == tag_pre;
       if (!stop && !blocked) {
@@ -934,7 +954,10 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
   }
   uint32_t ttot;
   const uint32_t texcl = block_excl_scan<kScanThreads>(cnt, ttot, s_scan);
-  if (t == 0) s_prefix = lookback_single(status, tile, ttot, &prm->error);
+  if (t < 64) {
+    const uint64_t pre = lookback_wave(status, tile, ttot, &prm->error);
+    if (t == 0) s_prefix = pre;
+  }
   __syncthreads();
   int32_t c = static_cast<int32_t>(s_prefix + texcl) - 1;
   int32_t pc[kScanItems];
@@ -1134,9 +1157,6 @@ __device__ __forceinline__ int32_t bcast(int32_t v, int src) {
 }
 __device__ __forceinline__ float bcastf(float v, int src) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-__device__ __forceinline__ int lowest_lane(uint64_t m) {
-  return __ffsll(static_cast<unsigned long long>(m)) - 1;
 }
 
 // Per-cell bounding boxes of the non-NaN points (the reference's per-cell
@@ -1725,7 +1745,10 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
   }
   uint32_t ttot;
   const uint32_t texcl = block_excl_scan<kScanThreads>(cnt, ttot, s_scan);
-  if (t == 0) s_prefix = lookback_single(status, tile, ttot, &prm->error);
+  if (t < 64) {
+    const uint64_t pre = lookback_wave(status, tile, ttot, &prm->error);
+    if (t == 0) s_prefix = pre;
+  }
   __syncthreads();
   uint32_t run = static_cast<uint32_t>(s_prefix) + texcl;
 #pragma unroll
