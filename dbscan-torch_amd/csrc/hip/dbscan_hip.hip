@@ -58,7 +58,7 @@ constexpr int32_t kNoise = -1;
 constexpr int kRadixBitsMax = 8;
 constexpr int kRadixMax = 1 << kRadixBitsMax;
 constexpr int kMaxPasses = 8;           // 64-bit keys at most
-constexpr int kSortThreads = 256;
+constexpr int kSortThreads = 512;
 // A radix pass ranks tiles of kSortThreads * items points, items being one of
 // 8, 12, 16 (the configuration's sort_items); status words are sized for the
 // smallest tile.
@@ -77,8 +77,9 @@ constexpr int kMaxReduceBlocks = 512;
 constexpr uint32_t kSpinLimit = 1u << 26;         // bounded waits; see ErrFlag
 constexpr uint64_t kCountMask = (1ull << 48) - 1;
 
-constexpr int kDigitsPerThread = kRadixMax / kSortThreads;
-static_assert(kDigitsPerThread * kSortThreads == kRadixMax, "digits split evenly over threads");
+// Thread t < kRadixMax of a sort workgroup owns digits [t*D, (t+1)*D).
+constexpr int kDigitsPerThread = kRadixMax > kSortThreads ? kRadixMax / kSortThreads : 1;
+static_assert(kDigitsPerThread * kSortThreads >= kRadixMax, "every digit has an owner");
 
 // Error bits left in Params::error when a bounded wait gives up (never
 // expected; checked by the tests through the debug layout).
@@ -710,7 +711,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
   if (pass >= m.npasses) return;
   constexpr int D = kDigitsPerThread;
 
-  __shared__ uint32_t s_buf[kTile];
+  __shared__ uint64_t s_buf[kTile];  // (x, y) pairs, then indices in its first half
   __shared__ uint16_t s_d[kTile];
   __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadixMax];
   __shared__ uint32_t s_goff[kRadixMax];
@@ -725,7 +726,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     uint32_t h[D], sum = 0;
 #pragma unroll
     for (int q = 0; q < D; ++q) {
-      h[q] = hist[t * D + q];
+      h[q] = t * D + q < kRadixMax ? hist[t * D + q] : 0u;
       sum += h[q];
     }
     uint32_t tot;
@@ -791,14 +792,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 #pragma unroll
   for (int q = 0; q < D; ++q) {
     const int d = t * D + q;
-    const uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
-    dtot[q] = c0 + c1 + c2 + c3;
-    s_wcnt[0][d] = 0u;
-    s_wcnt[1][d] = c0;
-    s_wcnt[2][d] = c0 + c1;
-    s_wcnt[3][d] = c0 + c1 + c2;
-    st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
-               (tile == first ? tag_pre : tag_agg) | dtot[q]);
+    dtot[q] = 0;
+    if (d < kRadixMax) {
+      // per-wave counts -> exclusive offsets of the waves
+#pragma unroll
+      for (int w = 0; w < kSortThreads / 64; ++w) {
+        const uint32_t c = s_wcnt[w][d];
+        s_wcnt[w][d] = dtot[q];
+        dtot[q] += c;
+      }
+      st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
+                 (tile == first ? tag_pre : tag_agg) | dtot[q]);
+    }
     dsum += dtot[q];
   }
   uint32_t ttot;
@@ -806,6 +811,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
 #pragma unroll
   for (int q = 0; q < D; ++q) {
     const int d = t * D + q;
+    if (d >= kRadixMax) break;
     uint64_t excl = 0;
     if (tile > first) {
       excl = digit_lookback(status, static_cast<int64_t>(tile), static_cast<int64_t>(first), d,
@@ -825,25 +831,44 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
     if (wl + k * 64 < tile_n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
   }
-  // x, y and index in turn through the one buffer
+  // The output position of each element this thread writes (element j = t +
+  // kSortThreads q of the tile, in digit order), then (x, y) and the index through the
+  // one LDS buffer.
 #pragma unroll
-  for (int arr = 0; arr < 3; ++arr) {
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      if (wl + k * 64 < tile_n)
-        s_buf[rk[k]] = arr == 0 ? __float_as_uint(x[k]) : (arr == 1 ? __float_as_uint(y[k]) : id[k]);
-    }
-    __syncthreads();
-    uint32_t* __restrict__ out = arr == 0 ? reinterpret_cast<uint32_t*>(out_x)
-                                          : (arr == 1 ? reinterpret_cast<uint32_t*>(out_y) : out_i);
-#pragma unroll 2
-    for (int j = t; j < static_cast<int>(tile_n); j += kSortThreads) {
-      const uint32_t pos = s_goff[s_d[j]] + static_cast<uint32_t>(j);
-      if (pos < n) out[pos] = s_buf[j];
-      else if (arr == 0) atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
-    }
-    __syncthreads();
+  for (int k = 0; k < kItems; ++k) {
+    if (wl + k * 64 < tile_n)
+      s_buf[rk[k]] = (static_cast<uint64_t>(__float_as_uint(y[k])) << 32) | __float_as_uint(x[k]);
   }
+  __syncthreads();
+  uint32_t pos[kItems];
+#pragma unroll
+  for (int q = 0; q < kItems; ++q) {
+    const int j = t + q * kSortThreads;
+    pos[q] = j < static_cast<int>(tile_n) ? s_goff[s_d[j]] + static_cast<uint32_t>(j) : n;
+  }
+#pragma unroll
+  for (int q = 0; q < kItems; ++q) {
+    const int j = t + q * kSortThreads;
+    if (pos[q] < n) {
+      const uint64_t v = s_buf[j];
+      out_x[pos[q]] = __uint_as_float(static_cast<uint32_t>(v));
+      out_y[pos[q]] = __uint_as_float(static_cast<uint32_t>(v >> 32));
+    } else if (j < static_cast<int>(tile_n)) {
+      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+    }
+  }
+  __syncthreads();
+  uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_buf);
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (wl + k * 64 < tile_n) s_idx[rk[k]] = id[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kItems; ++q) {
+    if (pos[q] < n) out_i[pos[q]] = s_idx[t + q * kSortThreads];
+  }
+  __syncthreads();
 }
 
 // Cell runs over the sorted points (dbscan_grid.hpp:206-227): flags where the
@@ -2068,10 +2093,10 @@ constexpr int kStageFirstKernel[DBSCAN_HIP_NUM_STAGES + 1] = {
 // Built-in launch configuration by input size: the autotuner's table for
 // MI355X (dbscan_torch/tuning_gfx950.json); a caller can pass its own.
 dbscan_hip_config default_config(int64_t n) {
-  if (n <= 31622) return dbscan_hip_config{8, 8, 1024, 4096};
-  if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 2048};
-  if (n <= 3162277) return dbscan_hip_config{8, 8, 16384, 16384};
-  return dbscan_hip_config{12, 8, 4096, 8192};
+  if (n <= 31622) return dbscan_hip_config{8, 7, 2048, 16384};
+  if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 4096};
+  if (n <= 3162277) return dbscan_hip_config{8, 8, 2048, 65536};
+  return dbscan_hip_config{12, 8, 8192, 8192};
 }
 
 bool config_ok(const dbscan_hip_config& c) {

@@ -3,7 +3,7 @@
 No counterpart in the reference (its launch shapes are fixed).  A
 configuration is (sort_items, radix_bits, cell_blocks, pair_blocks):
 
-  sort_items   points per thread of a radix-sort tile (tile = 256 x items):
+  sort_items   points per thread of a radix-sort tile (tile = 512 x items):
                8, 12 or 16
   radix_bits   widest radix digit the sort may use when it saves no pass
   cell_blocks  workgroups of the per-cell kernels (core marking, border
