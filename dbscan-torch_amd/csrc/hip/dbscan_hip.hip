@@ -1187,13 +1187,19 @@ __device__ __forceinline__ float bcastf(float v, int src) {
 // Per-cell bounding boxes of the non-NaN points (the reference's per-cell
 // boxes, dbscan_grid.hpp:332-349, used only for sound prunes), and core
 // marking pass 1 (dbscan_grid.hpp:270-279): every point of a cell with at
-// least min_samples points is core.  One thread per sorted point; each wave
-// reduces its runs of equal cells with shuffles.  A cell that lies wholly
-// inside the wave is stored directly; a run that touches either end of the
-// wave is merged with atomics into the sentinel written by k_cells.
+// least min_samples points is core.  Each lane holds one point of each of
+// kBoxChunks consecutive runs of 64 sorted points; the runs of equal cells
+// in a chunk are reduced with shuffles.  A cell that lies wholly inside a
+// chunk is stored directly; a run that touches either end of the chunk is
+// merged with atomics into the sentinel written by k_cells.
 // cell_flags bits: kCellHasNaN and, for full cells, kCellHasCore are set by
 // k_bbox; k_core_sparse sets the core bits of the other cells.
 constexpr uint32_t kCellHasCore = 1u, kCellHasNonCore = 2u, kCellHasNaN = 4u;
+
+// Each wave takes kBoxChunks consecutive runs of 64 points (lane = point in
+// each), with all their loads issued before the first reduction.
+constexpr int kBoxChunks = 4;
+constexpr int kBoxPointsPerBlock = 256 * kBoxChunks;
 
 template <class PP>
 __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
@@ -1207,60 +1213,75 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               uint32_t* __restrict__ cell_flags,
                                               uint8_t* __restrict__ is_core, int64_t n,
                                               PP pp) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  const bool valid = i < n;
-  int32_t c = -1;
-  uint32_t mnx = 0xFFFFFFFFu, mxx = 0u, mny = 0xFFFFFFFFu, mxy = 0u;
-  int32_t has_nan = 0;
-  bool full = false;
-  if (valid) {
-    c = point_cell[i];
-    full = cell_start[c + 1] - cell_start[c] >= pp.at(c).ms;
-    is_core[i] = full ? 1 : 0;  // points of other cells are decided by k_core_sparse
-    const float x = xs[i], y = ys[i];
-    if (!isnan(x) && !isnan(y)) {
-      mnx = mxx = f2ord(x);
-      mny = mxy = f2ord(y);
-    } else {
-      has_nan = 1;
-    }
-  }
-  // Suffix reduction within runs of equal c (runs are contiguous: sorted).
+  const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kBoxPointsPerBlock +
+                        static_cast<int64_t>(threadIdx.x >> 6) * (64 * kBoxChunks);
+  int32_t cc[kBoxChunks];
+  float px[kBoxChunks], py[kBoxChunks];
+  bool full[kBoxChunks];
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int32_t oc = __shfl_down(c, d, 64);
-    const uint32_t a0 = __shfl_down(mnx, d, 64), a1 = __shfl_down(mxx, d, 64);
-    const uint32_t a2 = __shfl_down(mny, d, 64), a3 = __shfl_down(mxy, d, 64);
-    const int32_t a4 = __shfl_down(has_nan, d, 64);
-    if (lane + d < 64 && oc == c) {
-      mnx = min(mnx, a0); mxx = max(mxx, a1);
-      mny = min(mny, a2); mxy = max(mxy, a3);
-      has_nan |= a4;
-    }
+  for (int k = 0; k < kBoxChunks; ++k) {
+    const int64_t i = wbase + k * 64 + lane;
+    const int64_t ic = i < n ? i : n - 1;  // clamped: defined, never used
+    cc[k] = point_cell[ic];
+    px[k] = xs[ic];
+    py[k] = ys[ic];
   }
-  const int32_t prev_c = __shfl_up(c, 1, 64);
-  const bool head = valid && (lane == 0 || prev_c != c);
-  // Lane of the last point of each run: the run ends inside the wave if the
-  // next lane holds another cell (or is past n).
-  const int32_t next_c = __shfl_down(c, 1, 64);
-  const bool tail = valid && (lane == 63 || next_c != c);
-  const uint64_t tails = __ballot(tail);
-  if (head && (has_nan || full))
-    atomicOr(cell_flags + c, (has_nan ? kCellHasNaN : 0u) | (full ? kCellHasCore : 0u));
-  if (head) {
-    // the run's last lane: first tail at or after this lane
-    const uint64_t after = tails & ~((1ull << lane) - 1ull);
-    const int last = lowest_lane(after);
-    const bool starts_here = lane > 0 || i == 0;
-    const bool ends_here = last < 63 || i + (last - lane) == n - 1;
-    if (starts_here && ends_here) {
-      cell_box[c] = make_uint4(mnx, mxx, mny, mxy);
-    } else {
-      if (mnx != 0xFFFFFFFFu) {
+#pragma unroll
+  for (int k = 0; k < kBoxChunks; ++k) {
+    const int64_t i = wbase + k * 64 + lane;
+    full[k] = cell_start[cc[k] + 1] - cell_start[cc[k]] >= pp.at(cc[k]).ms;
+    if (i < n) is_core[i] = full[k] ? 1 : 0;  // other cells' points: k_core_sparse
+  }
+#pragma unroll
+  for (int k = 0; k < kBoxChunks; ++k) {
+    const int64_t i = wbase + k * 64 + lane;
+    const bool valid = i < n;
+    const int32_t c = valid ? cc[k] : -1;
+    uint32_t mnx = 0xFFFFFFFFu, mxx = 0u, mny = 0xFFFFFFFFu, mxy = 0u;
+    int32_t has_nan = 0;
+    if (valid) {
+      if (!isnan(px[k]) && !isnan(py[k])) {
+        mnx = mxx = f2ord(px[k]);
+        mny = mxy = f2ord(py[k]);
+      } else {
+        has_nan = 1;
+      }
+    }
+    // Suffix reduction within runs of equal c (runs are contiguous: sorted).
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t oc = __shfl_down(c, d, 64);
+      const uint32_t a0 = __shfl_down(mnx, d, 64), a1 = __shfl_down(mxx, d, 64);
+      const uint32_t a2 = __shfl_down(mny, d, 64), a3 = __shfl_down(mxy, d, 64);
+      const int32_t a4 = __shfl_down(has_nan, d, 64);
+      if (lane + d < 64 && oc == c) {
+        mnx = min(mnx, a0); mxx = max(mxx, a1);
+        mny = min(mny, a2); mxy = max(mxy, a3);
+        has_nan |= a4;
+      }
+    }
+    const int32_t prev_c = __shfl_up(c, 1, 64);
+    const bool head = valid && (lane == 0 || prev_c != c);
+    // Lane of the last point of each run: the run ends inside the chunk if
+    // the next lane holds another cell (or is past n).
+    const int32_t next_c = __shfl_down(c, 1, 64);
+    const bool tail = valid && (lane == 63 || next_c != c);
+    const uint64_t tails = __ballot(tail);
+    if (head && (has_nan || full[k]))
+      atomicOr(cell_flags + c, (has_nan ? kCellHasNaN : 0u) | (full[k] ? kCellHasCore : 0u));
+    if (head) {
+      // the run's last lane: first tail at or after this lane
+      const uint64_t after = tails & ~((1ull << lane) - 1ull);
+      const int last = lowest_lane(after);
+      const bool starts_here = lane > 0 || i == 0;
+      const bool ends_here = last < 63 || i + (last - lane) == n - 1;
+      if (starts_here && ends_here) {
+        cell_box[c] = make_uint4(mnx, mxx, mny, mxy);
+      } else if (mnx != 0xFFFFFFFFu) {
         atomicMin(&cell_box[c].x, mnx);
         atomicMax(&cell_box[c].y, mxx);
         atomicMin(&cell_box[c].z, mny);
@@ -1506,37 +1527,88 @@ __global__ __launch_bounds__(256) void k_core_sparse(
     const float eps_sq = cp.eps_sq;
     const bool sparse = valid && e - s < min_samples;
     const bool whole_wave = sparse && e - s > 64;
-    uint64_t pending = 0;
     uint32_t bits = 0;
+    bool lonely = false;  // fewer than min_samples points in the whole window
     if (sparse && !whole_wave) {
       const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-      for (int32_t p = s; p < e; ++p) {
-        bool deferred = false;
-        const int32_t tot = lane_core_count(xs, ys, xs[p], ys[p], c, row, cell_start, cell_flags,
-                                            cell_box, min_samples, eps_sq, e - s, deferred);
-        if (tot >= min_samples) {
-          is_core[p] = 1;
-          bits |= kCellHasCore;
-        } else if (deferred) {
-          pending |= 1ull << (p - s);
-        } else {
-          is_core[p] = 0;
-          bits |= kCellHasNonCore;
+      // A point's count never exceeds the points of its 25 window cells, the
+      // five column runs [j0, j1): if they hold fewer than min_samples, no
+      // point of the cell is core.  (Exact, and one round of loads.)
+      int32_t j[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) j[k] = row[k];
+      int32_t wt = 0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        wt += j[2 * k + 1] > j[2 * k] ? cell_start[j[2 * k + 1]] - cell_start[j[2 * k]] : 0;
+      lonely = wt < min_samples;
+      if (lonely) {
+        for (int32_t p = s; p < e; ++p) is_core[p] = 0;
+        bits = kCellHasNonCore;
+      }
+    }
+    // The points of the cells still undecided are shared out one per lane:
+    // work item q of the wave is point q - excl of the cell of the lane whose
+    // [excl, incl) holds q (a binary search over the lanes' prefix sums).
+    const uint32_t cnt = (sparse && !whole_wave && !lonely) ? static_cast<uint32_t>(e - s) : 0u;
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    for (uint32_t r = 0; r < total; r += 64) {
+      const uint32_t q = r + static_cast<uint32_t>(lane);
+      int o = 0;  // owner lane: the number of lanes with incl <= q
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(incl), o + step - 1, 64));
+        if (v <= q) o += step;
+      }
+      o = min(o, 63);
+      const int32_t oc = __shfl(c, o, 64), os = __shfl(s, o, 64), oe = __shfl(e, o, 64);
+      const uint32_t oex = static_cast<uint32_t>(__shfl(static_cast<int>(incl - cnt), o, 64));
+      const int32_t oms = __shfl(min_samples, o, 64);
+      const float oeps = __shfl(eps_sq, o, 64);
+      bool deferred = false;
+      int32_t p = 0, tot = 0;
+      if (q < total) {
+        p = os + static_cast<int32_t>(q - oex);
+        const int32_t* row = ngh + static_cast<int64_t>(oc) * 10;
+        tot = lane_core_count(xs, ys, xs[p], ys[p], oc, row, cell_start, cell_flags, cell_box,
+                              oms, oeps, oe - os, deferred);
+        if (!deferred || tot >= oms) {
+          const bool core = tot >= oms;
+          is_core[p] = core ? 1 : 0;
+          atomicOr(cell_flags + oc, core ? kCellHasCore : kCellHasNonCore);
+          deferred = false;
+        }
+      }
+      // points whose count needs a scan of a big cell: the whole wave, one by one
+      uint64_t dm = __ballot(deferred);
+      while (dm) {
+        const int src = lowest_lane(dm);
+        dm &= dm - 1ull;
+        const int32_t pc = bcast(oc, src), pp_ = bcast(p, src);
+        const int32_t pms = bcast(oms, src);
+        const float peps = bcastf(oeps, src);
+        const int32_t* row = ngh + static_cast<int64_t>(pc) * 10;
+        const int32_t wt = wave_core_count(xs, ys, xs[pp_], ys[pp_], pc, row, cell_start,
+                                           cell_flags, cell_box, pms, peps,
+                                           bcast(oe - os, src));
+        if (lane == src) {
+          const bool core = wt >= pms;
+          is_core[pp_] = core ? 1 : 0;
+          atomicOr(cell_flags + pc, core ? kCellHasCore : kCellHasNonCore);
         }
       }
     }
-    uint64_t todo = __ballot(pending != 0 || whole_wave);
+    // cells of more than 64 points (min_samples > 64): the whole wave
+    uint64_t todo = __ballot(whole_wave);
     while (todo) {
       const int src = lowest_lane(todo);
       todo &= todo - 1ull;
       const int32_t cc = bcast(c, src), cs = bcast(s, src), ce = bcast(e, src);
       const int32_t wms = bcast(min_samples, src);
       const float weps = bcastf(eps_sq, src);
-      const uint64_t pm = bcast64(pending, src);
-      const bool all = ce - cs > 64;
       const int32_t* row = ngh + static_cast<int64_t>(cc) * 10;
       for (int32_t p = cs; p < ce; ++p) {
-        if (!all && !((pm >> (p - cs)) & 1ull)) continue;
         const int32_t tot = wave_core_count(xs, ys, xs[p], ys[p], cc, row, cell_start,
                                             cell_flags, cell_box, wms, weps, ce - cs);
         if (lane == src) {
@@ -1546,7 +1618,7 @@ __global__ __launch_bounds__(256) void k_core_sparse(
         }
       }
     }
-    if (sparse) cell_flags[c] |= bits;  // the lane owns this cell's core bits
+    if (lonely || whole_wave) cell_flags[c] |= bits;  // the lane owns these cells' bits
   }
 }
 
@@ -2243,7 +2315,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
       cell_key, cell_flags, cell_box, at<int2>(ws, L.col_range), L.max_cols, n, inv_r, nullptr,
       nullptr, nullptr);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
-  k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
+  k_bbox<<<grid_for(n, kBoxPointsPerBlock, INT_MAX), 256, 0, stream>>>(
+      prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
                                       cell_box, cell_flags, is_core, n, pp);
 
   // Stage 4: window columns.
@@ -2449,7 +2522,8 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
       cell_key, cell_flags, cell_box, col_range, L.max_cols, n, 0.f, seg_scan, clouds,
       cell_cloud);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
-  k_bbox<<<g_point, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
+  k_bbox<<<grid_for(n, kBoxPointsPerBlock, INT_MAX), 256, 0, stream>>>(
+      prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
                                       cell_box, cell_flags, is_core, n, pp);
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
   k_ngh<true><<<g_cells, 256, 0, stream>>>(prm, cell_key, col_range, L.max_cols, ngh, cell_ref,
