@@ -1487,11 +1487,6 @@ __device__ int32_t wave_core_count(const float* __restrict__ xs, const float* __
   return total;
 }
 
-__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
-  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), src);
-  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), src);
-  return (static_cast<uint64_t>(hi) << 32) | lo;
-}
 
 // Core marking pass 2 (dbscan_grid.hpp:281-307) for the cells with fewer than
 // min_samples points (pass 1 was done by k_bbox).  One lane per cell counts
@@ -1973,10 +1968,10 @@ __device__ int32_t wave_border(const float* __restrict__ xs, const float* __rest
 }
 
 // Border assignment (dbscan_grid.hpp:415-446) for the cells that hold a
-// non-core point: one lane per cell searches for each of its non-core points
-// in turn and writes the label to the point's input position.  Points whose
-// search needs a big cell, and cells of more than 64 points, are searched by
-// the whole wave.
+// non-core point: the points of the wave's cells are shared out one per lane;
+// a lane searches for its non-core point and writes the label to the point's
+// input position.  Points whose search needs a big cell are searched by the
+// whole wave.
 template <class PP>
 __global__ __launch_bounds__(256) void k_border(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
@@ -2007,38 +2002,51 @@ __global__ __launch_bounds__(256) void k_border(
       cp = pp.at(c);
     }
     const float eps_sq = cp.eps_sq, best_init = cp.best_init;
-    const bool whole_wave = has && e - s > 64;
-    uint64_t pending = 0;
-    if (has && !whole_wave) {
-      const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
-      for (int32_t p = s; p < e; ++p) {
-        if (is_core[p]) continue;
-        bool deferred = false;
-        const int32_t lab = lane_border(xs, ys, is_core, xs[p], ys[p], row, cell_start, cell_flags,
-                                        cell_box, cell_cluster, eps_sq, best_init, deferred);
-        if (deferred) {
-          pending |= 1ull << (p - s);
-        } else {
-          const uint32_t dst = idx[p];
-          if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
+    // The points of these cells are shared out one per lane (work item q of
+    // the wave is point q - excl of the cell of the lane whose [excl, incl)
+    // holds q); core points are skipped.
+    const uint32_t cnt = has ? static_cast<uint32_t>(e - s) : 0u;
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+    for (uint32_t r = 0; r < total; r += 64) {
+      const uint32_t q = r + static_cast<uint32_t>(lane);
+      int o = 0;  // owner lane: the number of lanes with incl <= q
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(incl), o + step - 1, 64));
+        if (v <= q) o += step;
+      }
+      o = min(o, 63);
+      const int32_t oc = __shfl(c, o, 64), os = __shfl(s, o, 64);
+      const uint32_t oex = static_cast<uint32_t>(__shfl(static_cast<int>(incl - cnt), o, 64));
+      const float oeps = __shfl(eps_sq, o, 64), obest = __shfl(best_init, o, 64);
+      bool deferred = false;
+      int32_t p = 0;
+      if (q < total) {
+        p = os + static_cast<int32_t>(q - oex);
+        if (!is_core[p]) {
+          const int32_t* row = ngh + static_cast<int64_t>(oc) * 10;
+          const int32_t lab = lane_border(xs, ys, is_core, xs[p], ys[p], row, cell_start,
+                                          cell_flags, cell_box, cell_cluster, oeps, obest,
+                                          deferred);
+          if (!deferred) {
+            const uint32_t dst = idx[p];
+            if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
+          }
         }
       }
-    }
-    uint64_t todo = __ballot(pending != 0 || whole_wave);
-    while (todo) {
-      const int src = lowest_lane(todo);
-      todo &= todo - 1ull;
-      const int32_t cc = bcast(c, src), cs = bcast(s, src), ce = bcast(e, src);
-      const float weps = bcastf(eps_sq, src), wbest = bcastf(best_init, src);
-      const uint64_t pm = bcast64(pending, src);
-      const bool all = ce - cs > 64;
-      const int32_t* row = ngh + static_cast<int64_t>(cc) * 10;
-      for (int32_t p = cs; p < ce; ++p) {
-        if (all ? is_core[p] != 0 : !((pm >> (p - cs)) & 1ull)) continue;
-        const int32_t lab = wave_border(xs, ys, is_core, xs[p], ys[p], row, cell_start,
-                                        cell_flags, cell_box, cell_cluster, weps, wbest);
+      // points whose search needs a big cell: the whole wave, one by one
+      uint64_t dm = __ballot(deferred);
+      while (dm) {
+        const int src = lowest_lane(dm);
+        dm &= dm - 1ull;
+        const int32_t pc = bcast(oc, src), pp_ = bcast(p, src);
+        const int32_t* row = ngh + static_cast<int64_t>(pc) * 10;
+        const int32_t lab = wave_border(xs, ys, is_core, xs[pp_], ys[pp_], row, cell_start,
+                                        cell_flags, cell_box, cell_cluster, bcastf(oeps, src),
+                                        bcastf(obest, src));
         if (lane == src) {
-          const uint32_t dst = idx[p];
+          const uint32_t dst = idx[pp_];
           if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
         }
       }
