@@ -1357,6 +1357,38 @@ __device__ __forceinline__ uint32_t lane_min_core_d2(const float* __restrict__ x
 // A cell pair whose point-count product is at most this is tested by one lane.
 constexpr int32_t kLanePairs = 1024;
 
+// A first try for a bigger pair by one lane: the first kSampleSide points of
+// each cell against each other (cores only).  A hit is a core pair within eps,
+// which decides the pair; otherwise the whole wave tests it.  Adjacent cells
+// inside a dense region nearly always connect at the first try.
+constexpr int kSampleSide = 4;
+
+__device__ __forceinline__ bool core_pair_sample(const float* __restrict__ xs,
+                                                 const float* __restrict__ ys,
+                                                 const uint8_t* __restrict__ is_core, int32_t sa,
+                                                 int32_t ea, int32_t sb, int32_t eb,
+                                                 float eps_sq) {
+  float ax[kSampleSide], ay[kSampleSide], bx[kSampleSide], by[kSampleSide];
+  bool ac[kSampleSide], bc[kSampleSide];
+#pragma unroll
+  for (int k = 0; k < kSampleSide; ++k) {
+    const int32_t a = sa + k < ea ? sa + k : ea - 1;  // clamped: repeats the last point
+    const int32_t b = sb + k < eb ? sb + k : eb - 1;
+    ax[k] = xs[a]; ay[k] = ys[a]; ac[k] = is_core[a] != 0;
+    bx[k] = xs[b]; by[k] = ys[b]; bc[k] = is_core[b] != 0;
+  }
+  bool hit = false;
+#pragma unroll
+  for (int i = 0; i < kSampleSide; ++i) {
+#pragma unroll
+    for (int k = 0; k < kSampleSide; ++k) {
+      const float ex = bx[k] - ax[i], ey = by[k] - ay[i];
+      hit |= ac[i] && bc[k] && ex * ex + ey * ey <= eps_sq;
+    }
+  }
+  return hit;
+}
+
 // Largest computed squared distance from (x, y) to any point of the box:
 // |fl(q - x)| is at most |fl(b - x)| for the box edge b on q's side (monotone
 // rounding), so every point of the box has d2 <= this value.  Valid only for
@@ -1774,6 +1806,8 @@ __global__ __launch_bounds__(256) void k_union(
         if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs) {
           if (core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq))
             uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
+        } else if (core_pair_sample(xs, ys, is_core, sa, ea, sb, eb, eps_sq)) {
+          uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
         } else {
           deferred = true;
         }
