@@ -82,6 +82,34 @@ def kernel_groups(names, npasses):
     return groups
 
 
+PROFILE_MD = os.path.join(ROOT, "profiles", "blobs10m_kernel_trace_and_hbm.md")
+
+
+def measured_traffic(group, npasses):
+    """Mean HBM bytes per launch of a kernel group, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE table committed for the blobs10m workload
+    (calibrated against a plain copy), or None without that table."""
+    if not os.path.exists(PROFILE_MD):
+        return None
+    rows = {}
+    for line in open(PROFILE_MD):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0].startswith("k_"):
+            try:
+                rows[cells[0]] = float(cells[5]) * 1e6
+            except ValueError:
+                pass
+    if group.startswith("k_sort_pass"):
+        first = group == "k_sort_pass<first>"
+        vals = [v for k, v in rows.items() if k.startswith("k_sort_pass<") and
+                (k.startswith("k_sort_pass<true") if first else
+                 (k.startswith("k_sort_pass<false") and
+                  1 <= int(k[k.rindex("[") + 1:-1]) < npasses))]
+    else:
+        vals = [v for k, v in rows.items() if k == group or k.startswith(group + "<")]
+    return sum(vals) / len(vals) if vals else None
+
+
 def cpu_baseline(X, eps, ms, reps=3):
     import numpy as np
     import torch
@@ -346,7 +374,11 @@ def main() -> int:
                    "sort_passes": npasses},
         "roofline": {"bound": "hbm", "kernel": dom,
                      "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": (measured_traffic(dom, npasses)
+                                 if args.workload == "blobs10m" else None),
+                     "traffic_source": "profiles/blobs10m_kernel_trace_and_hbm.md (rocprofv3 "
+                                       "FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated)",
                      "bytes_per_launch": int(d["bytes_per_launch"]),
                      "ms_per_launch": round(d["ms_per_launch"], 4)},
         "e2e_roofline": {"bytes_model": "107*N + 40*C", "bytes": e2e_bytes,
