@@ -675,207 +675,212 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 // then reordered by digit through one LDS buffer, array by array, so that runs
 // of equal digits leave as contiguous writes.  Thread t owns digits
 // [t*kDigitsPerThread, (t+1)*kDigitsPerThread).
-// kSeg (batches): tile `blockIdx.x` is tiles[blockIdx.x], a run of one
+// kSeg (batches): tile t is tiles[t], a run of one
 // cloud's points; the cloud's own histogram, key map and scale apply, the
 // look-back stops at the cloud's first tile, and the cloud's points stay in
 // its own range [begin, end), so every cloud is sorted on its own.
-template <bool kFirst, bool kSeg, int kItems>
+template <bool kFirst, bool kSeg, int kItems, bool kWalk>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
     int pass, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
-    const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist) {
+    const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist,
+    uint32_t num_tiles) {
   constexpr int kTile = kSortThreads * kItems;
   const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31: 32-bit offsets
-  const uint32_t tile = blockIdx.x;  // see kTileOrder
-  KeyMap m;
-  uint32_t base, tile_n, out_base = 0, first = 0;
-  const uint32_t* __restrict__ hist;
-  if constexpr (kSeg) {
-    const SegTile st = tiles[tile];
-    const CloudInfo& ci = clouds[st.cloud];
-    m = cloud_keymap(ci, prm->npasses);
-    base = static_cast<uint32_t>(st.start);
-    tile_n = static_cast<uint32_t>(st.count);
-    out_base = static_cast<uint32_t>(ci.begin);
-    first = static_cast<uint32_t>(st.first);
-    inv_r = ci.inv_r;
-    hist = cloud_hist + (static_cast<size_t>(st.cloud) * kMaxPasses + pass) * kRadixMax;
-  } else {
-    m = load_keymap(prm);
-    base = tile * kTile;
-    tile_n = min(static_cast<uint32_t>(kTile), n - base);
-    hist = prm->hist[pass];
-  }
-  if (pass >= m.npasses) return;
-  constexpr int D = kDigitsPerThread;
-
-  __shared__ uint64_t s_buf[kTile];  // (x, y) pairs, then indices in its first half
-  __shared__ uint16_t s_d[kTile];
-  __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadixMax];
-  __shared__ uint32_t s_goff[kRadixMax];
-  __shared__ uint32_t s_dstart[kRadixMax];
-  __shared__ uint32_t s_scan[kSortThreads / 64];
-
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int e = t; e < (kSortThreads / 64) * kRadixMax; e += kSortThreads) (&s_wcnt[0][0])[e] = 0u;
-  // global start of each owned digit: exclusive scan of the pass histogram
-  uint32_t goff[D];
-  {
-    uint32_t h[D], sum = 0;
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      h[q] = t * D + q < kRadixMax ? hist[t * D + q] : 0u;
-      sum += h[q];
-    }
-    uint32_t tot;
-    uint32_t run = out_base + block_excl_scan<kSortThreads>(sum, tot, s_scan);
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      goff[q] = run;
-      run += h[q];
-    }
-  }
-  // local index of item k of this lane: wl + k * 64
-  const uint32_t wl = static_cast<uint32_t>(wave) * (64 * kItems) + lane;
-
-  float x[kItems], y[kItems];
-  uint32_t id[kItems], dg[kItems], rk[kItems];
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    // unconditional loads (a clamped index past the end) keep the per-item
-    // values in plain registers; validity is tested where it matters
-    const uint32_t l = wl + k * 64;
-    const uint32_t e = base + l;
-    const uint32_t ec = l < tile_n ? e : base + tile_n - 1;
-    if (kFirst) {
-      const float2 p = X[ec];
-      x[k] = p.x; y[k] = p.y; id[k] = e;
+  if (pass >= prm->npasses) return;
+  // kWalk: a small grid walks the tiles (see launch_sort); otherwise the
+  // workgroup takes tile blockIdx.x only.
+  for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {  // see kTileOrder
+    KeyMap m;
+    uint32_t base, tile_n, out_base = 0, first = 0;
+    const uint32_t* __restrict__ hist;
+    if constexpr (kSeg) {
+      const SegTile st = tiles[tile];
+      const CloudInfo& ci = clouds[st.cloud];
+      m = cloud_keymap(ci, prm->npasses);
+      base = static_cast<uint32_t>(st.start);
+      tile_n = static_cast<uint32_t>(st.count);
+      out_base = static_cast<uint32_t>(ci.begin);
+      first = static_cast<uint32_t>(st.first);
+      inv_r = ci.inv_r;
+      hist = cloud_hist + (static_cast<size_t>(st.cloud) * kMaxPasses + pass) * kRadixMax;
     } else {
-      x[k] = in_x[ec]; y[k] = in_y[ec]; id[k] = in_i[ec];
+      m = load_keymap(prm);
+      base = tile * kTile;
+      tile_n = min(static_cast<uint32_t>(kTile), n - base);
+      hist = prm->hist[pass];
     }
-  }
-  // all loads of the tile are in flight before the first key is computed
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) dg[k] = key_digit(sort_key(x[k], y[k], inv_r, m), pass, m);
+    constexpr int D = kDigitsPerThread;
 
-  // Rank within the wave's segment, item by item (stable): lanes holding the
-  // same digit are found with one ballot per digit bit; the lowest such lane
-  // bumps the wave's counter for that digit.
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
+    __shared__ uint64_t s_buf[kTile];  // (x, y) pairs, then indices in its first half
+    __shared__ uint16_t s_d[kTile];
+    __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadixMax];
+    __shared__ uint32_t s_goff[kRadixMax];
+    __shared__ uint32_t s_dstart[kRadixMax];
+    __shared__ uint32_t s_scan[kSortThreads / 64];
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int e = t; e < (kSortThreads / 64) * kRadixMax; e += kSortThreads) (&s_wcnt[0][0])[e] = 0u;
+    // global start of each owned digit: exclusive scan of the pass histogram
+    uint32_t goff[D];
+    {
+      uint32_t h[D], sum = 0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const bool valid = wl + k * 64 < tile_n;
-    uint64_t peers = __ballot(valid);
+      for (int q = 0; q < D; ++q) {
+        h[q] = t * D + q < kRadixMax ? hist[t * D + q] : 0u;
+        sum += h[q];
+      }
+      uint32_t tot;
+      uint32_t run = out_base + block_excl_scan<kSortThreads>(sum, tot, s_scan);
 #pragma unroll
-    for (int b = 0; b < kRadixBitsMax; ++b) {
-      if (static_cast<uint32_t>(b) < m.dbits) {
-        const bool bit = (dg[k] >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
+      for (int q = 0; q < D; ++q) {
+        goff[q] = run;
+        run += h[q];
       }
     }
-    const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
-    uint32_t old = 0;
-    if (valid) old = s_wcnt[wave][dg[k]];
-    if (valid && below == 0) s_wcnt[wave][dg[k]] = old + static_cast<uint32_t>(__popcll(peers));
-    rk[k] = old + below;
-  }
-  __syncthreads();
+    // local index of item k of this lane: wl + k * 64
+    const uint32_t wl = static_cast<uint32_t>(wave) * (64 * kItems) + lane;
 
-  // Owned digits: wave offsets, tile totals, aggregate publication.
-  const uint64_t tag_agg = static_cast<uint64_t>(2 * pass + 1) << 48;
-  const uint64_t tag_pre = static_cast<uint64_t>(2 * pass + 2) << 48;
-  uint32_t dtot[D], dsum = 0;
+    float x[kItems], y[kItems];
+    uint32_t id[kItems], dg[kItems], rk[kItems];
 #pragma unroll
-  for (int q = 0; q < D; ++q) {
-    const int d = t * D + q;
-    dtot[q] = 0;
-    if (d < kRadixMax) {
-      // per-wave counts -> exclusive offsets of the waves
-#pragma unroll
-      for (int w = 0; w < kSortThreads / 64; ++w) {
-        const uint32_t c = s_wcnt[w][d];
-        s_wcnt[w][d] = dtot[q];
-        dtot[q] += c;
+    for (int k = 0; k < kItems; ++k) {
+      // unconditional loads (a clamped index past the end) keep the per-item
+      // values in plain registers; validity is tested where it matters
+      const uint32_t l = wl + k * 64;
+      const uint32_t e = base + l;
+      const uint32_t ec = l < tile_n ? e : base + tile_n - 1;
+      if (kFirst) {
+        const float2 p = X[ec];
+        x[k] = p.x; y[k] = p.y; id[k] = e;
+      } else {
+        x[k] = in_x[ec]; y[k] = in_y[ec]; id[k] = in_i[ec];
       }
-      st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
-                 (tile == first ? tag_pre : tag_agg) | dtot[q]);
     }
-    dsum += dtot[q];
-  }
-  uint32_t ttot;
-  uint32_t dstart = block_excl_scan<kSortThreads>(dsum, ttot, s_scan);
+    // all loads of the tile are in flight before the first key is computed
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int q = 0; q < D; ++q) {
-    const int d = t * D + q;
-    if (d >= kRadixMax) break;
-    uint64_t excl = 0;
-    if (tile > first) {
-      excl = digit_lookback(status, static_cast<int64_t>(tile), static_cast<int64_t>(first), d,
-                            tag_agg, tag_pre, &prm->error);
-      st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
-    }
-    s_dstart[d] = dstart;
-    s_goff[d] = goff[q] + static_cast<uint32_t>(excl) - dstart;
-    dstart += dtot[q];
-  }
-  __syncthreads();
+    for (int k = 0; k < kItems; ++k) dg[k] = key_digit(sort_key(x[k], y[k], inv_r, m), pass, m);
 
-  // Tile-local position of every item (kept in rk), then the digits in
-  // position order.
+    // Rank within the wave's segment, item by item (stable): lanes holding the
+    // same digit are found with one ballot per digit bit; the lowest such lane
+    // bumps the wave's counter for that digit.
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
-    if (wl + k * 64 < tile_n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
-  }
-  // The output position of each element this thread writes (element j = t +
-  // kSortThreads q of the tile, in digit order), then (x, y) and the index through the
-  // one LDS buffer.
+    for (int k = 0; k < kItems; ++k) {
+      const bool valid = wl + k * 64 < tile_n;
+      uint64_t peers = __ballot(valid);
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    if (wl + k * 64 < tile_n)
-      s_buf[rk[k]] = (static_cast<uint64_t>(__float_as_uint(y[k])) << 32) | __float_as_uint(x[k]);
-  }
-  __syncthreads();
-  uint32_t pos[kItems];
-#pragma unroll
-  for (int q = 0; q < kItems; ++q) {
-    const int j = t + q * kSortThreads;
-    pos[q] = j < static_cast<int>(tile_n) ? s_goff[s_d[j]] + static_cast<uint32_t>(j) : n;
-  }
-#pragma unroll
-  for (int q = 0; q < kItems; ++q) {
-    const int j = t + q * kSortThreads;
-    if (pos[q] < n) {
-      const uint64_t v = s_buf[j];
-      out_x[pos[q]] = __uint_as_float(static_cast<uint32_t>(v));
-      out_y[pos[q]] = __uint_as_float(static_cast<uint32_t>(v >> 32));
-    } else if (j < static_cast<int>(tile_n)) {
-      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+      for (int b = 0; b < kRadixBitsMax; ++b) {
+        if (static_cast<uint32_t>(b) < m.dbits) {
+          const bool bit = (dg[k] >> b) & 1u;
+          const uint64_t bal = __ballot(bit);
+          peers &= bit ? bal : ~bal;
+        }
+      }
+      const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
+      uint32_t old = 0;
+      if (valid) old = s_wcnt[wave][dg[k]];
+      if (valid && below == 0) s_wcnt[wave][dg[k]] = old + static_cast<uint32_t>(__popcll(peers));
+      rk[k] = old + below;
     }
-  }
-  __syncthreads();
-  uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_buf);
+    __syncthreads();
+
+    // Owned digits: wave offsets, tile totals, aggregate publication.
+    const uint64_t tag_agg = static_cast<uint64_t>(2 * pass + 1) << 48;
+    const uint64_t tag_pre = static_cast<uint64_t>(2 * pass + 2) << 48;
+    uint32_t dtot[D], dsum = 0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    if (wl + k * 64 < tile_n) s_idx[rk[k]] = id[k];
-  }
-  __syncthreads();
+    for (int q = 0; q < D; ++q) {
+      const int d = t * D + q;
+      dtot[q] = 0;
+      if (d < kRadixMax) {
+        // per-wave counts -> exclusive offsets of the waves
 #pragma unroll
-  for (int q = 0; q < kItems; ++q) {
-    if (pos[q] < n) out_i[pos[q]] = s_idx[t + q * kSortThreads];
+        for (int w = 0; w < kSortThreads / 64; ++w) {
+          const uint32_t c = s_wcnt[w][d];
+          s_wcnt[w][d] = dtot[q];
+          dtot[q] += c;
+        }
+        st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
+                   (tile == first ? tag_pre : tag_agg) | dtot[q]);
+      }
+      dsum += dtot[q];
+    }
+    uint32_t ttot;
+    uint32_t dstart = block_excl_scan<kSortThreads>(dsum, ttot, s_scan);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const int d = t * D + q;
+      if (d >= kRadixMax) break;
+      uint64_t excl = 0;
+      if (tile > first) {
+        excl = digit_lookback(status, static_cast<int64_t>(tile), static_cast<int64_t>(first), d,
+                              tag_agg, tag_pre, &prm->error);
+        st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
+      }
+      s_dstart[d] = dstart;
+      s_goff[d] = goff[q] + static_cast<uint32_t>(excl) - dstart;
+      dstart += dtot[q];
+    }
+    __syncthreads();
+
+    // Tile-local position of every item (kept in rk), then the digits in
+    // position order.
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
+      if (wl + k * 64 < tile_n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
+    }
+    // The output position of each element this thread writes (element j = t +
+    // kSortThreads q of the tile, in digit order), then (x, y) and the index through the
+    // one LDS buffer.
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      if (wl + k * 64 < tile_n)
+        s_buf[rk[k]] = (static_cast<uint64_t>(__float_as_uint(y[k])) << 32) | __float_as_uint(x[k]);
+    }
+    __syncthreads();
+    uint32_t pos[kItems];
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+      const int j = t + q * kSortThreads;
+      pos[q] = j < static_cast<int>(tile_n) ? s_goff[s_d[j]] + static_cast<uint32_t>(j) : n;
+    }
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+      const int j = t + q * kSortThreads;
+      if (pos[q] < n) {
+        const uint64_t v = s_buf[j];
+        out_x[pos[q]] = __uint_as_float(static_cast<uint32_t>(v));
+        out_y[pos[q]] = __uint_as_float(static_cast<uint32_t>(v >> 32));
+      } else if (j < static_cast<int>(tile_n)) {
+        atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+      }
+    }
+    __syncthreads();
+    uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_buf);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      if (wl + k * 64 < tile_n) s_idx[rk[k]] = id[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+      if (pos[q] < n) out_i[pos[q]] = s_idx[t + q * kSortThreads];
+    }
+    __syncthreads();
+    if constexpr (!kWalk) break;
   }
-  __syncthreads();
 }
 
 // Cell runs over the sorted points (dbscan_grid.hpp:206-227): flags where the
 // key changes, block scan plus look-back, then point->cell, cell starts and
 // cell keys.  Blocked layout: thread t owns kScanItems consecutive points,
 // read and written with 16-byte vector accesses where aligned.
-// kSeg (batches): tile `blockIdx.x` is tiles[blockIdx.x], a run of one
+// kSeg (batches): tile t is tiles[t], a run of one
 // cloud's points; a cloud's first point always opens a cell, keys are the
 // cloud's own, and the cell numbering runs on across clouds.  Each cloud's
 // cell range, and each cell's cloud, are recorded.
@@ -2219,15 +2224,44 @@ bool config_ok(const dbscan_hip_config& c) {
          c.pair_blocks >= 64 && c.pair_blocks <= 65536;
 }
 
-// A radix pass with `items` points per thread.
+// Passes from kTailPass on are needed only for keys wider than 32 bits
+// (4 x 8-bit digits); they are launched with one workgroup per CU that walks
+// the tiles (kWalk), so that for the usual narrow key each is a small launch
+// rather than one workgroup per tile.  One per CU is always co-resident, which
+// the walk needs: a workgroup waits only on tiles of lower index, held by
+// workgroups that are resident or done.  (The walk is a separate
+// instantiation: its loop would cost the one-tile kernel registers.)
+constexpr int kTailPass = 4;
+
+int tail_grid(int tiles) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 1;
+  return std::min(tiles, cus);
+}
+
+// Radix pass `pass` with `items` points per thread over `tiles` tiles.
 template <bool kFirst, bool kSeg, class... A>
-void launch_sort(int items, int grid, hipStream_t st, A... a) {
+void launch_sort(int items, int pass, int tiles, hipStream_t st, A... a) {
+  const uint32_t nt = static_cast<uint32_t>(tiles);
+  if (!kFirst && pass >= kTailPass) {
+    const int g = tail_grid(tiles);
+    if (items == 8)
+      k_sort_pass<kFirst, kSeg, 8, true><<<g, kSortThreads, 0, st>>>(a..., nt);
+    else if (items == 16)
+      k_sort_pass<kFirst, kSeg, 16, true><<<g, kSortThreads, 0, st>>>(a..., nt);
+    else
+      k_sort_pass<kFirst, kSeg, 12, true><<<g, kSortThreads, 0, st>>>(a..., nt);
+    return;
+  }
   if (items == 8)
-    k_sort_pass<kFirst, kSeg, 8><<<grid, kSortThreads, 0, st>>>(a...);
+    k_sort_pass<kFirst, kSeg, 8, false><<<tiles, kSortThreads, 0, st>>>(a..., nt);
   else if (items == 16)
-    k_sort_pass<kFirst, kSeg, 16><<<grid, kSortThreads, 0, st>>>(a...);
+    k_sort_pass<kFirst, kSeg, 16, false><<<tiles, kSortThreads, 0, st>>>(a..., nt);
   else
-    k_sort_pass<kFirst, kSeg, 12><<<grid, kSortThreads, 0, st>>>(a...);
+    k_sort_pass<kFirst, kSeg, 12, false><<<tiles, kSortThreads, 0, st>>>(a..., nt);
 }
 
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
@@ -2338,14 +2372,14 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   // Stage 2: radix passes (sort + gather in one).
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  launch_sort<true, false>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
+  launch_sort<true, false>(cfg.sort_items, 0, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
                            ys_a, idx_a, prm, sort_status, 0, n, inv_r, nullptr, nullptr,
                            nullptr);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
     launch_sort<false, false>(
-        cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
+        cfg.sort_items, p, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
         from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
         from_a ? idx_b : idx_a, prm, sort_status, p, n, inv_r, nullptr, nullptr, nullptr);
   }
@@ -2547,14 +2581,14 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   k_hist_seg<<<g_hist, 256, 0, stream>>>(X2, seg_sort, g_sort, hist_per_block, clouds, prm,
                                          cloud_hist, col_range);
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  launch_sort<true, true>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
+  launch_sort<true, true>(cfg.sort_items, 0, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
                           ys_a, idx_a, prm, sort_status, 0, n, 0.f, seg_sort, clouds,
                           cloud_hist);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
     launch_sort<false, true>(
-        cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
+        cfg.sort_items, p, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
         from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
         from_a ? idx_b : idx_a, prm, sort_status, p, n, 0.f, seg_sort, clouds, cloud_hist);
   }
