@@ -1195,8 +1195,10 @@ __device__ __forceinline__ float bcastf(float v, int src) {
 // least min_samples points is core.  Each lane holds one point of each of
 // kBoxChunks consecutive runs of 64 sorted points; the runs of equal cells
 // in a chunk are reduced with shuffles.  A cell that lies wholly inside a
-// chunk is stored directly; a run that touches either end of the chunk is
-// merged with atomics into the sentinel written by k_cells.
+// chunk is stored directly; a run touching either end of the chunk is folded,
+// with atomics, into the sentinel written by k_cells, the runs covering whole
+// chunks first merged per block in LDS (a cell of 300k points then costs 4
+// atomics per 1024 points, not per 64).
 // cell_flags bits: kCellHasNaN and, for full cells, kCellHasCore are set by
 // k_bbox; k_core_sparse sets the core bits of the other cells.
 constexpr uint32_t kCellHasCore = 1u, kCellHasNonCore = 2u, kCellHasNaN = 4u;
@@ -1218,12 +1220,16 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               uint32_t* __restrict__ cell_flags,
                                               uint8_t* __restrict__ is_core, int64_t n,
                                               PP pp) {
-  const int lane = threadIdx.x & 63;
+  __shared__ int32_t s_pc[4 * kBoxChunks];  // whole-chunk runs: cell (or -1)
+  __shared__ uint4 s_pb[4 * kBoxChunks];     // and their boxes
+  __shared__ uint32_t s_pf[4 * kBoxChunks];  // and flag bits
+  bool any_whole = false;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
   const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kBoxPointsPerBlock +
-                        static_cast<int64_t>(threadIdx.x >> 6) * (64 * kBoxChunks);
+                        static_cast<int64_t>(wv) * (64 * kBoxChunks);
   int32_t cc[kBoxChunks];
   float px[kBoxChunks], py[kBoxChunks];
   bool full[kBoxChunks];
@@ -1276,8 +1282,13 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
     const int32_t next_c = __shfl_down(c, 1, 64);
     const bool tail = valid && (lane == 63 || next_c != c);
     const uint64_t tails = __ballot(tail);
-    if (head && (has_nan || full[k]))
-      atomicOr(cell_flags + c, (has_nan ? kCellHasNaN : 0u) | (full[k] ? kCellHasCore : 0u));
+    const uint32_t fl = (has_nan ? kCellHasNaN : 0u) | (full[k] ? kCellHasCore : 0u);
+    // A run wholly inside the chunk is stored; one touching an end of the
+    // chunk is folded in with atomics, except a run covering the whole chunk
+    // (a cell of 64 points or more), which waits in the chunk's LDS slot for
+    // the block's merge below.
+    const int slot = wv * kBoxChunks + k;
+    if (lane == 0) s_pc[slot] = -1;
     if (head) {
       // the run's last lane: first tail at or after this lane
       const uint64_t after = tails & ~((1ull << lane) - 1ull);
@@ -1286,13 +1297,54 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
       const bool ends_here = last < 63 || i + (last - lane) == n - 1;
       if (starts_here && ends_here) {
         cell_box[c] = make_uint4(mnx, mxx, mny, mxy);
-      } else if (mnx != 0xFFFFFFFFu) {
-        atomicMin(&cell_box[c].x, mnx);
-        atomicMax(&cell_box[c].y, mxx);
-        atomicMin(&cell_box[c].z, mny);
-        atomicMax(&cell_box[c].w, mxy);
+        if (fl) atomicOr(cell_flags + c, fl);
+      } else if (lane == 0 && last == 63) {
+        s_pc[slot] = c;
+        s_pb[slot] = make_uint4(mnx, mxx, mny, mxy);
+        s_pf[slot] = fl;
+        any_whole = true;
+      } else {
+        if (mnx != 0xFFFFFFFFu) {
+          atomicMin(&cell_box[c].x, mnx);
+          atomicMax(&cell_box[c].y, mxx);
+          atomicMin(&cell_box[c].z, mny);
+          atomicMax(&cell_box[c].w, mxy);
+        }
+        if (fl) atomicOr(cell_flags + c, fl);
       }
     }
+  }
+  // Whole-chunk runs of the block, in point order: equal neighbours are one
+  // cell, folded in once.  (Such a cell always reaches past these chunks.)
+  if (__syncthreads_or(any_whole) && threadIdx.x == 0) {
+    int32_t rc = -1;
+    uint4 rb = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
+    uint32_t rf = 0;
+    auto flush = [&]() {
+      if (rc < 0) return;
+      if (rb.x != 0xFFFFFFFFu) {
+        atomicMin(&cell_box[rc].x, rb.x);
+        atomicMax(&cell_box[rc].y, rb.y);
+        atomicMin(&cell_box[rc].z, rb.z);
+        atomicMax(&cell_box[rc].w, rb.w);
+      }
+      if (rf) atomicOr(cell_flags + rc, rf);
+    };
+    for (int q = 0; q < 4 * kBoxChunks; ++q) {
+      const int32_t qc = s_pc[q];
+      if (qc < 0) continue;
+      const uint4 qb = s_pb[q];
+      if (qc == rc) {
+        rb = make_uint4(min(rb.x, qb.x), max(rb.y, qb.y), min(rb.z, qb.z), max(rb.w, qb.w));
+        rf |= s_pf[q];
+        continue;
+      }
+      flush();
+      rc = qc;
+      rb = qb;
+      rf = s_pf[q];
+    }
+    flush();
   }
 }
 
