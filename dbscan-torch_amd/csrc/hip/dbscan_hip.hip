@@ -626,7 +626,7 @@ constexpr int kSortLookback = 4;
 __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ status,
                                                    int64_t tile, int64_t first, int digit,
                                                    uint64_t tag_agg, uint64_t tag_pre,
-                                                   int32_t* error) {
+                                                   int32_t* error, int stride = kRadixMax) {
   uint64_t excl = 0;
   int64_t j = tile - 1;
   uint32_t spins = 0;
@@ -635,7 +635,7 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 #pragma unroll
     for (int k = 0; k < kSortLookback; ++k)
       w[k] = (j - k >= first)
-                 ? ld_acquire(status + static_cast<uint64_t>(j - k) * kRadixMax + digit)
+                 ? ld_acquire(status + static_cast<uint64_t>(j - k) * stride + digit)
                  : tag_pre;
     // consume the published words in order: `used` of them, `stop` at a prefix
     int used = 0;
@@ -1964,19 +1964,147 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
   }
 }
 
-// Core labels and unpermute (dbscan_grid.hpp:401-412,448-455): a core point
-// takes its cell's cluster id, written to its input position.  Non-core
-// points are written by k_border.
-__global__ __launch_bounds__(256) void k_label(
+// Core labels and unpermute (dbscan_grid.hpp:401-412,448-455), in two
+// passes so that no label is written alone into a random cache line.
+// Buckets of 2^S consecutive input positions, S = kUnpermLdsBits, at most
+// kMaxBuckets of them (inputs up to 2^24 points; larger ones take
+// k_label_direct): bucket b receives exactly the points whose input position
+// is in [b 2^S, (b+1) 2^S), so its pair region starts at b 2^S.
+constexpr int kMaxBuckets = 512;
+constexpr int kUnpermLdsBits = 15;  // labels k_unpermute holds in LDS at once (128 KB)
+constexpr int kLabelThreads = 512;
+constexpr int kLabelItems = 16;
+constexpr int kLabelTile = kLabelThreads * kLabelItems;
+constexpr int kUnpermThreads = 1024;
+static_assert(kLabelThreads >= kMaxBuckets, "a thread per bucket");
+constexpr uint64_t kLabelTagAgg = 17ull << 48, kLabelTagPre = 18ull << 48;  // after the passes' tags
+
+// Pass A: every sorted point's label (a core point: its cell's cluster id;
+// another point: the border label k_border left in sorted order) and input
+// position go, as a pair, to the point's bucket.  As in a radix pass, the
+// tile's pairs are grouped by bucket in LDS and each bucket's run lands after
+// the earlier tiles' runs of that bucket (decoupled look-back, one bucket per
+// thread), so the runs leave as contiguous writes; the order inside a bucket
+// is free.
+__global__ __launch_bounds__(kLabelThreads) void k_label(
+    Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
+    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
+    const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
+    const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
+    uint32_t* __restrict__ spare_xa, uint32_t* __restrict__ spare_xb,
+    uint32_t* __restrict__ spare_ya, uint32_t* __restrict__ spare_yb,
+    uint64_t* __restrict__ status, int64_t n64, int S) {
+  __shared__ uint32_t s_cnt[kMaxBuckets];  // counts, then the runs' local starts
+  __shared__ uint32_t s_base[kMaxBuckets];
+  __shared__ uint64_t s_buf[kLabelTile];   // (label << 32 | input position) by bucket
+  __shared__ uint32_t s_scan[kLabelThreads / 64];
+  const uint32_t n = static_cast<uint32_t>(n64);
+  const bool in_a = (prm->npasses & 1) != 0;
+  const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
+  const int32_t* __restrict__ lab_sorted =
+      reinterpret_cast<const int32_t*>(in_a ? spare_ib : spare_ia);
+  uint32_t* __restrict__ pair_i = in_a ? spare_xb : spare_xa;
+  int32_t* __restrict__ pair_l = reinterpret_cast<int32_t*>(in_a ? spare_yb : spare_ya);
+  const int t = threadIdx.x;
+  const uint32_t nb = (n + (1u << S) - 1u) >> S;
+  const uint32_t tile = blockIdx.x;  // see kTileOrder
+  const uint32_t base = tile * kLabelTile;
+  const uint32_t tile_n = min(static_cast<uint32_t>(kLabelTile), n - base);
+  for (uint32_t e = t; e < nb; e += kLabelThreads) s_cnt[e] = 0u;
+  __syncthreads();
+  uint32_t dst[kLabelItems], rk[kLabelItems];
+  int32_t lab[kLabelItems];
+#pragma unroll
+  for (int k = 0; k < kLabelItems; ++k) {
+    const uint32_t l = k * kLabelThreads + t;
+    dst[k] = n;
+    if (l < tile_n) {
+      const uint32_t i = base + l;
+      dst[k] = idx[i];
+      lab[k] = is_core[i] ? cell_cluster[point_cell[i]] : lab_sorted[i];
+      if (dst[k] < n) rk[k] = atomicAdd(&s_cnt[dst[k] >> S], 1u);
+    }
+  }
+  __syncthreads();
+  const bool owner = static_cast<uint32_t>(t) < nb;
+  const uint32_t cnt = owner ? s_cnt[t] : 0u;
+  uint32_t ttot;
+  const uint32_t lstart = block_excl_scan<kLabelThreads>(cnt, ttot, s_scan);
+  if (owner) {
+    uint64_t excl = 0;
+    st_release(status + static_cast<uint64_t>(tile) * kMaxBuckets + t,
+               (tile == 0 ? kLabelTagPre : kLabelTagAgg) | cnt);
+    if (tile > 0) {
+      excl = digit_lookback(status, static_cast<int64_t>(tile), 0, t, kLabelTagAgg, kLabelTagPre,
+                            &prm->error, kMaxBuckets);
+      st_release(status + static_cast<uint64_t>(tile) * kMaxBuckets + t,
+                 kLabelTagPre | (excl + cnt));
+    }
+    s_base[t] = (static_cast<uint32_t>(t) << S) + static_cast<uint32_t>(excl) - lstart;
+    s_cnt[t] = lstart;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kLabelItems; ++k) {
+    if (dst[k] < n) {
+      s_buf[s_cnt[dst[k] >> S] + rk[k]] =
+          (static_cast<uint64_t>(static_cast<uint32_t>(lab[k])) << 32) | dst[k];
+    } else if (k * kLabelThreads + t < tile_n) {
+      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = t; j < tile_n; j += kLabelThreads) {
+    const uint64_t v = s_buf[j];
+    const uint32_t d = static_cast<uint32_t>(v), b = d >> S;
+    const uint32_t pos = s_base[b] + j;
+    if (pos < n && pos < ((b + 1u) << S)) {
+      pair_i[pos] = d;
+      pair_l[pos] = static_cast<int32_t>(v >> 32);
+    } else {
+      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+    }
+  }
+}
+
+// Inputs of more than kMaxBuckets << kUnpermLdsBits points: labels written
+// straight to their input positions (one 4-byte write per label).
+__global__ __launch_bounds__(256) void k_label_direct(
     const Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
     const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
     const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
+    const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
     int32_t* __restrict__ labels, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n || !is_core[i]) return;
-  const uint32_t* __restrict__ idx = (prm->npasses & 1) ? idx_a : idx_b;
-  const uint32_t dst = idx[i];
-  if (dst < static_cast<uint64_t>(n)) labels[dst] = cell_cluster[point_cell[i]];
+  if (i >= n) return;
+  const bool in_a = (prm->npasses & 1) != 0;
+  const uint32_t dst = (in_a ? idx_a : idx_b)[i];
+  const int32_t* __restrict__ lab_sorted =
+      reinterpret_cast<const int32_t*>(in_a ? spare_ib : spare_ia);
+  if (dst < static_cast<uint64_t>(n))
+    labels[dst] = is_core[i] ? cell_cluster[point_cell[i]] : lab_sorted[i];
+}
+
+// Pass B: one workgroup per bucket places the bucket's labels in LDS by input
+// position and writes them out in order.
+__global__ __launch_bounds__(kUnpermThreads) void k_unpermute(
+    const Params* __restrict__ prm, const uint32_t* __restrict__ spare_xa,
+    const uint32_t* __restrict__ spare_xb, const uint32_t* __restrict__ spare_ya,
+    const uint32_t* __restrict__ spare_yb, int32_t* __restrict__ labels, int64_t n64, int S) {
+  __shared__ int32_t s_lab[1 << kUnpermLdsBits];
+  const uint32_t n = static_cast<uint32_t>(n64);
+  const bool in_a = (prm->npasses & 1) != 0;
+  const uint32_t* __restrict__ pair_i = in_a ? spare_xb : spare_xa;
+  const int32_t* __restrict__ pair_l = reinterpret_cast<const int32_t*>(in_a ? spare_yb : spare_ya);
+  const int t = threadIdx.x;
+  const uint32_t b0 = static_cast<uint32_t>(blockIdx.x) << S;  // S == kUnpermLdsBits
+  const uint32_t b1 = min(n, b0 + (1u << S));
+  for (uint32_t j = b0 + t; j < b1; j += kUnpermThreads) {
+    const uint32_t d = pair_i[j];
+    if (d >= b0 && d < b1) s_lab[d - b0] = pair_l[j];
+  }
+  __syncthreads();
+  for (uint32_t j = b0 + t; j < b1; j += kUnpermThreads) labels[j] = s_lab[j - b0];
 }
 
 // Border search for one point by one lane (dbscan_grid.hpp:419-445): window
@@ -2060,23 +2188,23 @@ __device__ int32_t wave_border(const float* __restrict__ xs, const float* __rest
 
 // Border assignment (dbscan_grid.hpp:415-446) for the cells that hold a
 // non-core point: the points of the wave's cells are shared out one per lane;
-// a lane searches for its non-core point and writes the label to the point's
-// input position.  Points whose search needs a big cell are searched by the
-// whole wave.
+// a lane searches for its non-core point and writes the label in sorted order
+// into the index buffer the sort left spare (k_label moves it on).  Points
+// whose search needs a big cell are searched by the whole wave.
 template <class PP>
 __global__ __launch_bounds__(256) void k_border(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
-    const float* __restrict__ ys_a, const uint32_t* __restrict__ idx_a,
+    const float* __restrict__ ys_a, uint32_t* __restrict__ spare_ia,
     const float* __restrict__ xs_b, const float* __restrict__ ys_b,
-    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ cell_start,
+    uint32_t* __restrict__ spare_ib, const int32_t* __restrict__ cell_start,
     const int32_t* __restrict__ ngh, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const uint8_t* __restrict__ is_core,
-    const int32_t* __restrict__ cell_cluster, int32_t* __restrict__ labels, int64_t n, PP pp) {
+    const int32_t* __restrict__ cell_cluster, PP pp) {
   const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
+  int32_t* __restrict__ lab_sorted = reinterpret_cast<int32_t*>(in_a ? spare_ib : spare_ia);
   const int lane = threadIdx.x & 63;
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
@@ -2120,10 +2248,7 @@ __global__ __launch_bounds__(256) void k_border(
           const int32_t lab = lane_border(xs, ys, is_core, xs[p], ys[p], row, cell_start,
                                           cell_flags, cell_box, cell_cluster, oeps, obest,
                                           deferred);
-          if (!deferred) {
-            const uint32_t dst = idx[p];
-            if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
-          }
+          if (!deferred) lab_sorted[p] = lab;
         }
       }
       // points whose search needs a big cell: the whole wave, one by one
@@ -2136,10 +2261,7 @@ __global__ __launch_bounds__(256) void k_border(
         const int32_t lab = wave_border(xs, ys, is_core, xs[pp_], ys[pp_], row, cell_start,
                                         cell_flags, cell_box, cell_cluster, bcastf(oeps, src),
                                         bcastf(obest, src));
-        if (lane == src) {
-          const uint32_t dst = idx[pp_];
-          if (dst < static_cast<uint64_t>(n)) labels[dst] = lab;
-        }
+        if (lane == src) lab_sorted[pp_] = lab;
       }
     }
   }
@@ -2176,7 +2298,10 @@ Layout make_layout(int64_t n, int64_t clouds = 0) {
     return at;
   };
   L.params = take(sizeof(Params));
-  L.sort_status = take(sort_tiles * kRadixMax * sizeof(uint64_t));
+  // the radix passes' look-back words, reused by the label pass (k_label)
+  const size_t label_tiles = (N + kLabelTile - 1) / kLabelTile;
+  L.sort_status = take(std::max(sort_tiles * kRadixMax, label_tiles * kMaxBuckets) *
+                       sizeof(uint64_t));
   L.cell_status = take(scan_tiles * sizeof(uint64_t));
   L.rank_status = take(scan_tiles * sizeof(uint64_t));
   L.cloud_hist = take(B * kMaxPasses * kRadixMax * sizeof(uint32_t));
@@ -2216,6 +2341,15 @@ const char* const kStageNames[DBSCAN_HIP_NUM_STAGES] = {
     "cell_keys", "sort_gather", "boundary_scan", "build_ngh", "core_mark",
     "union_find", "cluster_id", "core_labels", "border_assign", "unpermute"};
 
+// Bucket width S of the label unpermute: 2^kUnpermLdsBits input positions
+// per bucket (one LDS load of k_unpermute), or 0 when that needs more than
+// kMaxBuckets buckets (k_label_direct then writes the labels).
+int bucket_bits(int64_t n) {
+  return ((n + (int64_t(1) << kUnpermLdsBits) - 1) >> kUnpermLdsBits) <= kMaxBuckets
+             ? kUnpermLdsBits
+             : 0;
+}
+
 int grid_for(int64_t items, int per_block, int cap) {
   int64_t g = (items + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -2244,8 +2378,9 @@ enum KernelSlot : int {
   kKFlatten,
   kKRank,
   kKCluster,
-  kKLabel,
   kKBorder,
+  kKLabel,
+  kKUnpermute,
   kNumKernels
 };
 static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header");
@@ -2253,13 +2388,15 @@ static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header
 const char* const kKernelNames[kNumKernels] = {
     "k_bounds", "k_hist", "k_sort_pass[0]", "k_sort_pass[1]", "k_sort_pass[2]",
     "k_sort_pass[3]", "k_sort_pass[4]", "k_sort_pass[5]", "k_sort_pass[6]", "k_sort_pass[7]",
-    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_flatten[3x3]", "k_union[rest]", "k_flatten", "k_rank", "k_cluster", "k_label", "k_border"};
+    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_flatten[3x3]", "k_union[rest]", "k_flatten", "k_rank", "k_cluster", "k_border", "k_label", "k_unpermute"};
 
-// First kernel of each reference stage (stages 7-9 and 9-10 share kernels:
-// core labels, border assignment and unpermute are one fused kernel).
-constexpr int kStageFirstKernel[DBSCAN_HIP_NUM_STAGES + 1] = {
-    kKBounds, kKSort0, kKCells, kKNgh, kKCore, kKUnion, kKRank, kKLabel, kKBorder,
-    kNumKernels, kNumKernels};
+// Kernel slots [begin, end) of each reference stage.  Border assignment runs
+// before the core labels here: it writes sorted-order labels that the label
+// pass (k_label: pairs into buckets) then moves with the core ones.
+constexpr int kStageSlots[DBSCAN_HIP_NUM_STAGES][2] = {
+    {kKBounds, kKSort0}, {kKSort0, kKCells},   {kKCells, kKNgh},        {kKNgh, kKCore},
+    {kKCore, kKUnion},   {kKUnion, kKRank},    {kKRank, kKBorder},      {kKLabel, kKUnpermute},
+    {kKBorder, kKLabel}, {kKUnpermute, kNumKernels}};
 
 // Built-in launch configuration by input size: the autotuner's table for
 // MI355X (dbscan_torch/tuning_gfx950.json); a caller can pass its own.
@@ -2343,7 +2480,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const float best_init = eps_sq + 1.0f;
   const OneCloud pp{eps_sq, best_init, ms};
 
-  hipEvent_t ev[DBSCAN_HIP_NUM_STAGES + 1] = {};
+  hipEvent_t ev[kNumKernels + 1] = {};  // profile: one before every kernel slot
   auto release = [&]() {
     for (auto& x : ev)
       if (x) (void)hipEventDestroy(x);
@@ -2363,9 +2500,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   auto before = [&](int k) -> hipError_t {
     hipError_t e = hipSuccess;
     if (kev) e = hipEventRecord(kev[k], stream);
-    if (stage_ms)
-      for (int s = 0; s <= DBSCAN_HIP_NUM_STAGES && e == hipSuccess; ++s)
-        if (kStageFirstKernel[s] == k) e = hipEventRecord(ev[s], stream);
+    if (stage_ms && e == hipSuccess) e = hipEventRecord(ev[k], stream);
     return e;
   };
   hipError_t e;
@@ -2376,8 +2511,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
     k_fill_i32<<<grid_for(n, 256, 4096), 256, 0, stream>>>(labels, n, kNoise);
     if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
-    if (stage_ms && (e = hipEventSynchronize(ev[DBSCAN_HIP_NUM_STAGES])) != hipSuccess)
-      return fail(e);
+    if (stage_ms && (e = hipEventSynchronize(ev[kNumKernels])) != hipSuccess) return fail(e);
     release();
     return DBSCAN_HIP_OK;
   }
@@ -2408,7 +2542,6 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_hist = grid_for(n, 4096, kMaxReduceBlocks);
   const int g_sort = grid_for(n, kSortThreads * cfg.sort_items, INT_MAX);
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
-  const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
   const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);  // one lane per cell, grid-stride
   const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
@@ -2476,21 +2609,38 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   k_cluster<false><<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank,
                                                 cell_cluster, nullptr, nullptr);
 
-  // Stages 8-10: core labels, border assignment and unpermute, fused.
-  if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  k_label<<<g_point, 256, 0, stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster,
-                                       labels, n);
+  // Stages 8-10: border assignment (sorted order), then core labels and the
+  // unpermute in two bucketed passes.
   if ((e = before(kKBorder)) != hipSuccess) return fail(e);
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
-                                            ngh, cell_flags, cell_box, is_core, cell_cluster,
-                                            labels, n, pp);
+                                            ngh, cell_flags, cell_box, is_core, cell_cluster, pp);
+  if ((e = before(kKLabel)) != hipSuccess) return fail(e);
+  const int S = bucket_bits(n);
+  if (S > 0) {
+    k_label<<<grid_for(n, kLabelTile, INT_MAX), kLabelThreads, 0, stream>>>(
+        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
+        reinterpret_cast<uint32_t*>(xs_a), reinterpret_cast<uint32_t*>(xs_b),
+        reinterpret_cast<uint32_t*>(ys_a), reinterpret_cast<uint32_t*>(ys_b), sort_status, n, S);
+  } else {
+    k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
+        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
+  }
+  if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
+  if (S > 0) {
+    k_unpermute<<<static_cast<int>((n + (int64_t(1) << S) - 1) >> S), kUnpermThreads, 0,
+                  stream>>>(prm, reinterpret_cast<uint32_t*>(xs_a),
+                            reinterpret_cast<uint32_t*>(xs_b), reinterpret_cast<uint32_t*>(ys_a),
+                            reinterpret_cast<uint32_t*>(ys_b), labels, n, S);
+  }
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
   if (stage_ms) {
-    if ((e = hipEventSynchronize(ev[DBSCAN_HIP_NUM_STAGES])) != hipSuccess) return fail(e);
+    if ((e = hipEventSynchronize(ev[kNumKernels])) != hipSuccess) return fail(e);
     for (int s = 0; s < DBSCAN_HIP_NUM_STAGES; ++s) {
-      if ((e = hipEventElapsedTime(&stage_ms[s], ev[s], ev[s + 1])) != hipSuccess) return fail(e);
+      if ((e = hipEventElapsedTime(&stage_ms[s], ev[kStageSlots[s][0]], ev[kStageSlots[s][1]])) !=
+          hipSuccess)
+        return fail(e);
     }
   }
   release();
@@ -2617,7 +2767,6 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const int g_sort = static_cast<int>(sort_tiles);
   const int g_scan_pts = static_cast<int>(scan_tiles);
   const int g_scan_cells = grid_for(n, kScanTile, INT_MAX);
-  const int g_point = grid_for(n, 256, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
   const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);
   const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
@@ -2674,13 +2823,27 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   if ((e = before(kKCluster)) != hipSuccess) return fail(e);
   k_cluster<true><<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank, cell_cluster,
                                                cell_cloud, clouds);
-  if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  k_label<<<g_point, 256, 0, stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster,
-                                       labels, n);
   if ((e = before(kKBorder)) != hipSuccess) return fail(e);
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
-                                            ngh, cell_flags, cell_box, is_core, cell_cluster,
-                                            labels, n, pp);
+                                            ngh, cell_flags, cell_box, is_core, cell_cluster, pp);
+  if ((e = before(kKLabel)) != hipSuccess) return fail(e);
+  const int S = bucket_bits(n);
+  if (S > 0) {
+    k_label<<<grid_for(n, kLabelTile, INT_MAX), kLabelThreads, 0, stream>>>(
+        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
+        reinterpret_cast<uint32_t*>(xs_a), reinterpret_cast<uint32_t*>(xs_b),
+        reinterpret_cast<uint32_t*>(ys_a), reinterpret_cast<uint32_t*>(ys_b), sort_status, n, S);
+  } else {
+    k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
+        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
+  }
+  if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
+  if (S > 0) {
+    k_unpermute<<<static_cast<int>((n + (int64_t(1) << S) - 1) >> S), kUnpermThreads, 0,
+                  stream>>>(prm, reinterpret_cast<uint32_t*>(xs_a),
+                            reinterpret_cast<uint32_t*>(xs_b), reinterpret_cast<uint32_t*>(ys_a),
+                            reinterpret_cast<uint32_t*>(ys_b), labels, n, S);
+  }
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   return DBSCAN_HIP_OK;

@@ -107,6 +107,23 @@ def test_large_clouds_in_a_batch(dt, golden_dir):
         assert np.array_equal(got[i], oracle.dbscan2d(clouds[i], eps[i], ms[i]))
 
 
+def test_twenty_million_point_batch(dt, golden_dir):
+    """blobs10m and skewed10m in one batch (past 2^24 points in all, where
+    labels go straight to their input positions) keep their reference
+    labels."""
+    with open(os.path.join(golden_dir, "golden.json")) as f:
+        golden = json.load(f)
+    rng = np.random.default_rng(78)
+    a, ea, ma = W.blobs10m()
+    b, eb, mb = W.skewed10m()
+    small = _cloud(rng, 5000)
+    got = batched(dt, [a, small, b], [ea, 0.5, eb], [ma, 5, mb])
+    for lab, name in ((got[0], "blobs10m"), (got[2], "skewed10m")):
+        h = hashlib.sha256(np.ascontiguousarray(lab, dtype=np.int32).tobytes()).hexdigest()
+        assert h == golden[name]["sha256"], name
+    assert np.array_equal(got[1], oracle.dbscan2d(small, 0.5, 5))
+
+
 def test_alone_and_in_the_mix(dt):
     """A cloud's labels do not depend on the other clouds of its batch nor on
     its position in it."""

@@ -108,6 +108,16 @@ def test_large_configs(dt, golden, name):
     assert error_flag(dt, X, eps, ms) == 0
 
 
+def test_twenty_million_points(dt):
+    """Past 2^24 points the labels go straight to their input positions
+    instead of through the bucketed two-pass placement: blobs10m and a copy
+    shifted by 4096 in x and y, against the oracle."""
+    X, eps, ms = W.blobs10m()
+    X2 = np.concatenate([X, X + np.float32(4096.0)])
+    labels = gpu_labels(dt, X2, eps, ms)
+    assert np.array_equal(labels, oracle.dbscan2d(X2, eps, ms))
+
+
 def test_permutation_equivariance_lattice1m(dt):
     X, eps, ms = W.lattice1m()
     base = gpu_labels(dt, X, eps, ms)
