@@ -53,6 +53,10 @@ HBM_PEAK_GBS = 8000.0
 #   k_cells         reads x, y (8), writes point_cell (4); start+key 12/cell
 #   k_bbox          reads x, y, point_cell (12), writes is_core (1) and a
 #                   16-byte box per cell
+#   k_union[3x3]    cell flags, starts, boxes (12 per cell, 9 per point of the
+#                   tested pairs), writes the cell's hit mask (4)
+#   k_union_local   reads the hit mask (4), window row (40), reference index
+#                   (4), writes uf (4) per cell
 KERNEL_BYTES = {
     "k_sort_pass[0]": (20, 0),
     "k_sort_pass[1]": (24, 0), "k_sort_pass[2]": (24, 0), "k_sort_pass[3]": (24, 0),
@@ -61,7 +65,8 @@ KERNEL_BYTES = {
     "k_bounds": (8, 0), "k_hist": (8, 0),
     "k_cells": (12, 12), "k_bbox": (13, 16), "k_core_sparse": (0, 12),
     "k_label": (17, 0), "k_unpermute": (12, 0), "k_border": (0, 12),
-    "k_ngh": (0, 48), "k_union[3x3]": (9, 12), "k_flatten[3x3]": (0, 12),
+    "k_ngh": (0, 48), "k_union[3x3]": (9, 16), "k_union_local": (0, 52),
+    "k_flatten[3x3]": (0, 12),
     "k_union[rest]": (0, 24), "k_flatten": (0, 12), "k_rank": (0, 8),
     "k_cluster": (0, 16),
 }

@@ -1046,6 +1046,13 @@ __device__ __forceinline__ int32_t lower_bound_u64(const uint64_t* __restrict__ 
   return lo;
 }
 
+// The x array of the radix buffer pair that the last pass did not write: free
+// from the cell pass until the border pass; k_ngh clears it and the 3x3 pair
+// pass records its hits there (a 7-bit slot mask per cell).
+__device__ __forceinline__ uint32_t* spare_words(const Params* prm, float* xs_a, float* xs_b) {
+  return reinterpret_cast<uint32_t*>((prm->npasses & 1) ? xs_b : xs_a);
+}
+
 // Window columns, reference index, union-find init.  For column x' + dx the
 // cells with y' in [y'-2, y'+2] are a contiguous run [j0, j1) of cells, so
 // the 25-cell window of the reference (dbscan_grid.hpp:239-252) is 5 runs,
@@ -1062,8 +1069,10 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
                                              int32_t* __restrict__ cell_ref,
                                              int32_t* __restrict__ uf,
                                              const int32_t* __restrict__ cell_cloud,
-                                             const CloudInfo* __restrict__ clouds) {
+                                             const CloudInfo* __restrict__ clouds,
+                                             float* __restrict__ xs_a, float* __restrict__ xs_b) {
   const int32_t C = prm->num_cells;
+  uint32_t* __restrict__ hits = spare_words(prm, xs_a, xs_b);
   for (int64_t c64 = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c64 < C;
        c64 += static_cast<int64_t>(gridDim.x) * 256) {
     const int32_t c = static_cast<int32_t>(c64);
@@ -1145,6 +1154,7 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
     }
     cell_ref[c] = r;
     uf[r] = r;
+    hits[c] = 0u;
   }
 }
 
@@ -1793,11 +1803,15 @@ __device__ __forceinline__ bool core_pair_within_lane(const float* __restrict__ 
 // on both sides and boxes within eps are tested by the lane when small, and by
 // the whole wave otherwise.
 // Two launches split the pairs: kPhase 0 takes the 3x3 neighbours (dx <= 1,
-// |dy| <= 1; only slots 0-5 can hold one), which join almost every dense
-// region; after a flatten, kPhase 1 takes the rest and first skips pairs whose
-// cells already share a root.  The union is order-independent, so the split
-// cannot change a label.
+// |dy| <= 1; only slots 0-6 can hold one), which join almost every dense
+// region, and only records each hit as bit `slot` of the cell's mask (the
+// unions follow in k_union_local, mostly in LDS); after a flatten, kPhase 1
+// takes the rest, first skipping pairs whose cells already share a root, and
+// unites in uf directly.  The union is order-independent, so the split cannot
+// change a label.
 constexpr int kUnionSlots = 12;
+// Cells per workgroup of k_union_local; its ranges start at multiples of it.
+constexpr int kUnionRange = 1024;
 
 // Forward neighbour `slot` of cell c from its window row, with its column
 // offset, or -1.  Slots 0-1: the next two cells of c's column; 2-6: the cells
@@ -1823,8 +1837,10 @@ __global__ __launch_bounds__(256) void k_union(
     const int32_t* __restrict__ cell_start, const int32_t* __restrict__ ngh,
     const int32_t* __restrict__ cell_ref, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const uint64_t* __restrict__ cell_key,
-    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, PP pp) {
+    const uint8_t* __restrict__ is_core, int32_t* __restrict__ uf, PP pp,
+    float* __restrict__ spare_xa, float* __restrict__ spare_xb) {
   constexpr int S = kPhase == 0 ? 7 : kUnionSlots;  // slots scanned by this phase
+  uint32_t* __restrict__ hits = spare_words(prm, spare_xa, spare_xb);
   const int64_t C = prm->num_cells;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
@@ -1836,9 +1852,10 @@ __global__ __launch_bounds__(256) void k_union(
   for (int64_t base = wave * 64; base < total; base += nwaves * 64) {
     const int64_t t = base + lane;
     int32_t c = -1, nc = -1;
+    int slot = 0;
     if (t < total) {
       c = static_cast<int32_t>(t / S);
-      const int slot = static_cast<int>(t - static_cast<int64_t>(c) * S);
+      slot = static_cast<int>(t - static_cast<int64_t>(c) * S);
       if (cell_flags[c] & kCellHasCore) {
         int dx;
         nc = forward_neighbour(ngh + static_cast<int64_t>(c) * 10, c, slot, dx);
@@ -1860,13 +1877,18 @@ __global__ __launch_bounds__(256) void k_union(
       if (box_box_d2(box_c, box_n) <= eps_sq) {
         const int32_t sa = cell_start[c], ea = cell_start[c + 1];
         const int32_t sb = cell_start[nc], eb = cell_start[nc + 1];
-        if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs) {
-          if (core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq))
-            uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
-        } else if (core_pair_sample(xs, ys, is_core, sa, ea, sb, eb, eps_sq)) {
-          uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
-        } else {
+        bool hit = false;
+        if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs)
+          hit = core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq);
+        else if (core_pair_sample(xs, ys, is_core, sa, ea, sb, eb, eps_sq))
+          hit = true;
+        else
           deferred = true;
+        if (hit) {
+          if constexpr (kPhase == 0)
+            atomicOr(hits + c, 1u << slot);
+          else
+            uf_unite(uf, cell_ref[c], cell_ref[nc], &prm->error);
         }
       }
     }
@@ -1874,15 +1896,126 @@ __global__ __launch_bounds__(256) void k_union(
     while (todo) {
       const int src = lowest_lane(todo);
       todo &= todo - 1ull;
-      const int32_t cw = bcast(c, src), nw = bcast(nc, src);
+      const int32_t cw = bcast(c, src), nw = bcast(nc, src), sw = bcast(slot, src);
       const float weps = bcastf(eps_sq, src);
       const int32_t rc = cell_ref[cw], rn = cell_ref[nw];
-      if (uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
+      if (kPhase == 1 && uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
       const bool hit = core_pair_within(xs, ys, is_core, cell_start[cw], cell_start[cw + 1],
                                         load_box(cell_box, cw), cell_start[nw],
                                         cell_start[nw + 1], load_box(cell_box, nw), weps);
-      if (hit && lane == 0) uf_unite(uf, rc, rn, &prm->error);
+      if (hit && lane == 0) {
+        if constexpr (kPhase == 0)
+          atomicOr(hits + cw, 1u << sw);
+        else
+          uf_unite(uf, rc, rn, &prm->error);
+      }
     }
+  }
+}
+
+// The unions of the 3x3 hits (masks from k_union<0>).  A workgroup takes a
+// range of kUnionRange consecutive cells (a few whole columns of a dense
+// region, so nearly every 3x3 neighbour is in the same range): hits between
+// two of its cells are unions in LDS (parents in local indices, hooked by
+// reference index, larger onto smaller).  The range's forest then goes to uf,
+// each linked cell onto its local root, by CAS (a cell that another
+// workgroup's cross-range union hooked meanwhile takes a union in uf
+// instead); only then are the hits reaching past the range united in uf, so
+// that their finds start from written forests and repeated hits between two
+// components end at the first step.  Every step links a root onto a smaller
+// reference index, so any interleaving gives the same components.  The pair
+// tests stay in k_union<0>, spread evenly over the grid; this pass only reads
+// masks and window rows.
+constexpr int kUnionLocalThreads = 512;
+constexpr int kCrossCap = 2048;
+
+__device__ __forceinline__ int32_t lds_find(int32_t* s_par, int32_t x) {
+  int32_t p = __hip_atomic_load(s_par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (p != x) {
+    const int32_t gp =
+        __hip_atomic_load(s_par + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (gp != p)
+      __hip_atomic_store(s_par + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__device__ void lds_unite(int32_t* s_par, const int32_t* s_ref, int32_t a, int32_t b,
+                          int32_t* error) {
+  for (uint32_t it = 0; it < kSpinLimit; ++it) {
+    a = lds_find(s_par, a);
+    b = lds_find(s_par, b);
+    if (a == b) return;
+    if (s_ref[a] < s_ref[b]) {
+      const int32_t tmp = a;
+      a = b;
+      b = tmp;
+    }
+    if (atomicCAS(s_par + a, a, b) == a) return;
+  }
+  atomicOr(error, static_cast<int32_t>(kErrUfLoop));
+}
+
+__global__ __launch_bounds__(kUnionLocalThreads) void k_union_local(
+    Params* __restrict__ prm, const int32_t* __restrict__ ngh,
+    const int32_t* __restrict__ cell_ref, int32_t* __restrict__ uf,
+    float* __restrict__ spare_xa, float* __restrict__ spare_xb) {
+  __shared__ int32_t s_par[kUnionRange];
+  __shared__ int32_t s_ref[kUnionRange];
+  __shared__ int2 s_cross[kCrossCap];  // (local cell, neighbour) of hits past the range
+  __shared__ int32_t s_ncross;
+  const uint32_t* __restrict__ hits = spare_words(prm, spare_xa, spare_xb);
+  const int64_t C = prm->num_cells;
+  const int t = threadIdx.x;
+  for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kUnionRange; c0 < C;
+       c0 += static_cast<int64_t>(gridDim.x) * kUnionRange) {
+    const int32_t nloc = static_cast<int32_t>(min<int64_t>(kUnionRange, C - c0));
+    const int64_t c1 = c0 + nloc;
+    for (int i = t; i < nloc; i += kUnionLocalThreads) {
+      s_par[i] = i;
+      s_ref[i] = cell_ref[c0 + i];
+    }
+    if (t == 0) s_ncross = 0;
+    __syncthreads();
+    for (int i = t; i < nloc; i += kUnionLocalThreads) {
+      const int32_t c = static_cast<int32_t>(c0) + i;
+      uint32_t m = hits[c];
+      while (m) {
+        const int slot = __builtin_ctz(m);
+        m &= m - 1u;
+        int dx;
+        const int32_t nc = forward_neighbour(ngh + static_cast<int64_t>(c) * 10, c, slot, dx);
+        if (nc < 0) {  // a hit is only recorded for an existing neighbour
+          atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+        } else if (nc < c1) {
+          lds_unite(s_par, s_ref, i, nc - static_cast<int32_t>(c0), &prm->error);
+        } else {
+          const int32_t q = atomicAdd(&s_ncross, 1);
+          if (q < kCrossCap)
+            s_cross[q] = make_int2(i, nc);
+          else  // list full: united now (correct in any order, only slower)
+            uf_unite(uf, s_ref[i], cell_ref[nc], &prm->error);
+        }
+      }
+    }
+    __syncthreads();
+    // the range's forest to uf: each linked cell onto its local root
+    for (int i = t; i < nloc; i += kUnionLocalThreads) {
+      const int32_t r = lds_find(s_par, i);
+      if (r != i) {
+        const int32_t gx = s_ref[i], gy = s_ref[r];
+        if (atomicCAS(uf + gx, gx, gy) != gx) uf_unite(uf, gx, gy, &prm->error);
+      }
+    }
+    __syncthreads();
+    const int32_t ncross = min(s_ncross, kCrossCap);
+    for (int q = t; q < ncross; q += kUnionLocalThreads) {
+      const int2 h = s_cross[q];
+      uf_unite(uf, s_ref[h.x], cell_ref[h.y], &prm->error);
+    }
+    __syncthreads();
   }
 }
 
@@ -2373,6 +2506,7 @@ enum KernelSlot : int {
   kKNgh,
   kKCore,
   kKUnion,
+  kKUnionLocal,
   kKFlatten0,
   kKUnion1,
   kKFlatten,
@@ -2388,7 +2522,7 @@ static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header
 const char* const kKernelNames[kNumKernels] = {
     "k_bounds", "k_hist", "k_sort_pass[0]", "k_sort_pass[1]", "k_sort_pass[2]",
     "k_sort_pass[3]", "k_sort_pass[4]", "k_sort_pass[5]", "k_sort_pass[6]", "k_sort_pass[7]",
-    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_flatten[3x3]", "k_union[rest]", "k_flatten", "k_rank", "k_cluster", "k_border", "k_label", "k_unpermute"};
+    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_union_local", "k_flatten[3x3]", "k_union[rest]", "k_flatten", "k_rank", "k_cluster", "k_border", "k_label", "k_unpermute"};
 
 // Kernel slots [begin, end) of each reference stage.  Border assignment runs
 // before the core labels here: it writes sorted-order labels that the label
@@ -2545,6 +2679,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_cells = grid_for(n, 256, 16384);
   const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);  // one lane per cell, grid-stride
   const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
+  const int g_union_local = grid_for(n, kUnionRange, 4096);
 
   // Stage 1: cell keys (bounds + digit histograms).
   if ((e = before(kKBounds)) != hipSuccess) return fail(e);
@@ -2583,7 +2718,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   // Stage 4: window columns.
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
   k_ngh<false><<<g_cells, 256, 0, stream>>>(prm, cell_key, at<int2>(ws, L.col_range),
-                                            L.max_cols, ngh, cell_ref, uf, nullptr, nullptr);
+                                            L.max_cols, ngh, cell_ref, uf, nullptr, nullptr,
+                                            xs_a, xs_b);
 
   // Stage 5: core marking.
   if ((e = before(kKCore)) != hipSuccess) return fail(e);
@@ -2593,12 +2729,17 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   // Stage 6: union-find.
   if ((e = before(kKUnion)) != hipSuccess) return fail(e);
   k_union<0><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp, xs_a,
+                                          xs_b);
+  if ((e = before(kKUnionLocal)) != hipSuccess) return fail(e);
+  k_union_local<<<g_union_local, kUnionLocalThreads, 0, stream>>>(prm, ngh, cell_ref, uf, xs_a,
+                                                                  xs_b);
   if ((e = before(kKFlatten0)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
   if ((e = before(kKUnion1)) != hipSuccess) return fail(e);
   k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp, xs_a,
+                                          xs_b);
   if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
 
@@ -2770,6 +2911,7 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const int g_cells = grid_for(n, 256, 16384);
   const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);
   const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
+  const int g_union_local = grid_for(n, kUnionRange, 4096);
   const int hist_per_block = static_cast<int>((sort_tiles + 1023) / 1024);
   const int g_hist = static_cast<int>((sort_tiles + hist_per_block - 1) / hist_per_block);
 
@@ -2804,18 +2946,23 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
                                       cell_box, cell_flags, is_core, n, pp);
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);
   k_ngh<true><<<g_cells, 256, 0, stream>>>(prm, cell_key, col_range, L.max_cols, ngh, cell_ref,
-                                           uf, cell_cloud, clouds);
+                                           uf, cell_cloud, clouds, xs_a, xs_b);
   if ((e = before(kKCore)) != hipSuccess) return fail(e);
   k_core_sparse<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh,
                                                  cell_box, is_core, cell_flags, pp);
   if ((e = before(kKUnion)) != hipSuccess) return fail(e);
   k_union<0><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp, xs_a,
+                                          xs_b);
+  if ((e = before(kKUnionLocal)) != hipSuccess) return fail(e);
+  k_union_local<<<g_union_local, kUnionLocalThreads, 0, stream>>>(prm, ngh, cell_ref, uf, xs_a,
+                                                                  xs_b);
   if ((e = before(kKFlatten0)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
   if ((e = before(kKUnion1)) != hipSuccess) return fail(e);
   k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
-                                          cell_flags, cell_box, cell_key, is_core, uf, pp);
+                                          cell_flags, cell_box, cell_key, is_core, uf, pp, xs_a,
+                                          xs_b);
   if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
   k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
   if ((e = before(kKRank)) != hipSuccess) return fail(e);
