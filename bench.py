@@ -45,8 +45,9 @@ HBM_PEAK_GBS = 8000.0
 #   k_sort_pass[0]  reads X (8) and writes x, y, index (12)
 #   k_sort_pass[p]  reads and writes x, y, index (12 + 12)
 #   k_label         reads index (4), is_core (1), point_cell (4), writes the
-#                   (input position, label) pair into its bucket (8)
-#   k_unpermute     reads the pairs (8), writes the labels in input order (4)
+#                   (position, label) pair into its bucket (4: one packed word
+#                   below 131,071 clusters, as for blobs10m)
+#   k_unpermute     reads the pairs (4), writes the labels in input order (4)
 #   k_border        per cell flags and start (12); window reads not counted
 #   k_core_sparse   reads cell_start and flags per cell (12); window reads of
 #                   sparse cells' points are not counted
@@ -64,7 +65,7 @@ KERNEL_BYTES = {
     "k_sort_pass[7]": (24, 0),
     "k_bounds": (8, 0), "k_hist": (8, 0),
     "k_cells": (12, 12), "k_bbox": (13, 16), "k_core_sparse": (0, 12),
-    "k_label": (17, 0), "k_unpermute": (12, 0), "k_border": (0, 12),
+    "k_label": (13, 0), "k_unpermute": (8, 0), "k_border": (0, 12),
     "k_ngh": (0, 48), "k_union[3x3]": (9, 16), "k_union_local": (0, 52),
     "k_flatten[3x3]": (0, 12),
     "k_union[rest]": (0, 24), "k_flatten": (0, 12), "k_rank": (0, 8),

@@ -42,6 +42,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "dbscan_hip.h"
@@ -98,6 +99,7 @@ struct Params {
   int32_t npasses;
   int32_t error;
   int32_t col_total;  // batches: column-index entries in use
+  int32_t num_roots;  // roots over all cells (k_rank): bounds every cluster id
   // Everything below is cleared by k_bounds at the start of every call.
   uint32_t tile_counter[16];
   uint32_t hist[kMaxPasses][kRadixMax];
@@ -2071,6 +2073,7 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
   for (int j = 0; j < kScanItems; ++j) {
     if (e0 + j < C) flag_rank[e0 + j] = run;
     run += f[j];
+    if (e0 + j == C - 1) prm->num_roots = static_cast<int32_t>(run);
   }
 }
 
@@ -2099,17 +2102,30 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
 
 // Core labels and unpermute (dbscan_grid.hpp:401-412,448-455), in two
 // passes so that no label is written alone into a random cache line.
-// Buckets of 2^S consecutive input positions, S = kUnpermLdsBits, at most
-// kMaxBuckets of them (inputs up to 2^24 points; larger ones take
-// k_label_direct): bucket b receives exactly the points whose input position
-// is in [b 2^S, (b+1) 2^S), so its pair region starts at b 2^S.
+// Buckets of B consecutive input positions (bucket_width(): at most
+// kMaxBuckets buckets of at most kUnpermMaxWidth positions, so inputs up to
+// 2^24 points; larger ones take k_label_direct): bucket b receives exactly the
+// points whose input position is in [b B, (b+1) B), so its pair region starts
+// at b B.  B is the smallest multiple of 256 (at least 4,096) that needs no
+// more than kMaxBuckets buckets: for 10M points 508 buckets of 80 KB of
+// labels, two k_unpermute workgroups per CU, one round over the chip.
 constexpr int kMaxBuckets = 512;
-constexpr int kUnpermLdsBits = 15;  // labels k_unpermute holds in LDS at once (128 KB)
-constexpr int kLabelThreads = 512;
+constexpr int kUnpermMaxWidth = 1 << 15;  // labels k_unpermute holds in LDS at once (128 KB)
+constexpr int kUnpermSmallWidth = 20480;  // up to here the 80 KB instantiation
 constexpr int kLabelItems = 16;
-constexpr int kLabelTile = kLabelThreads * kLabelItems;
-constexpr int kUnpermThreads = 1024;
-static_assert(kLabelThreads >= kMaxBuckets, "a thread per bucket");
+constexpr int kLabelTile = 512 * kLabelItems;  // the smaller tile of the two forms
+// With fewer than kPackedRoots roots (so every label + 1 < 2^17) a pair is
+// one word: (label + 1) << 15 | position within the bucket (B <= 2^15); else
+// two words, input position and label.  The form is read from num_roots on
+// the device; both forms are launched and the other returns at once.
+constexpr int32_t kPackedRoots = (1 << 17) - 1;
+template <bool kPacked>
+struct LabelForm {
+  static constexpr int kThreads = kPacked ? 1024 : 512;
+  static constexpr int kTile = kThreads * kLabelItems;
+  using Slot = std::conditional_t<kPacked, uint32_t, uint64_t>;  // 64 KB of LDS either way
+};
+static_assert(LabelForm<false>::kThreads >= kMaxBuckets, "a thread per bucket");
 constexpr uint64_t kLabelTagAgg = 17ull << 48, kLabelTagPre = 18ull << 48;  // after the passes' tags
 
 // Pass A: every sorted point's label (a core point: its cell's cluster id;
@@ -2117,20 +2133,25 @@ constexpr uint64_t kLabelTagAgg = 17ull << 48, kLabelTagPre = 18ull << 48;  // a
 // position go, as a pair, to the point's bucket.  As in a radix pass, the
 // tile's pairs are grouped by bucket in LDS and each bucket's run lands after
 // the earlier tiles' runs of that bucket (decoupled look-back, one bucket per
-// thread), so the runs leave as contiguous writes; the order inside a bucket
-// is free.
-__global__ __launch_bounds__(kLabelThreads) void k_label(
+// thread), so the runs leave as contiguous writes (16 lanes per run); the
+// order inside a bucket is free.
+template <bool kPacked>
+__global__ __launch_bounds__(LabelForm<kPacked>::kThreads) void k_label(
     Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
     const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
     const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
     const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
     uint32_t* __restrict__ spare_xa, uint32_t* __restrict__ spare_xb,
     uint32_t* __restrict__ spare_ya, uint32_t* __restrict__ spare_yb,
-    uint64_t* __restrict__ status, int64_t n64, int S) {
+    uint64_t* __restrict__ status, int64_t n64, uint32_t B) {
+  using F = LabelForm<kPacked>;
+  constexpr int T = F::kThreads;
   __shared__ uint32_t s_cnt[kMaxBuckets];  // counts, then the runs' local starts
-  __shared__ uint32_t s_base[kMaxBuckets];
-  __shared__ uint64_t s_buf[kLabelTile];   // (label << 32 | input position) by bucket
-  __shared__ uint32_t s_scan[kLabelThreads / 64];
+  __shared__ uint32_t s_len[kMaxBuckets];
+  __shared__ uint32_t s_base[kMaxBuckets];  // the runs' global starts
+  __shared__ typename F::Slot s_buf[F::kTile];
+  __shared__ uint32_t s_scan[T / 64];
+  if ((prm->num_roots < kPackedRoots) != kPacked) return;  // the other form's launch
   const uint32_t n = static_cast<uint32_t>(n64);
   const bool in_a = (prm->npasses & 1) != 0;
   const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
@@ -2139,30 +2160,30 @@ __global__ __launch_bounds__(kLabelThreads) void k_label(
   uint32_t* __restrict__ pair_i = in_a ? spare_xb : spare_xa;
   int32_t* __restrict__ pair_l = reinterpret_cast<int32_t*>(in_a ? spare_yb : spare_ya);
   const int t = threadIdx.x;
-  const uint32_t nb = (n + (1u << S) - 1u) >> S;
+  const uint32_t nb = (n + B - 1u) / B;
   const uint32_t tile = blockIdx.x;  // see kTileOrder
-  const uint32_t base = tile * kLabelTile;
-  const uint32_t tile_n = min(static_cast<uint32_t>(kLabelTile), n - base);
-  for (uint32_t e = t; e < nb; e += kLabelThreads) s_cnt[e] = 0u;
+  const uint32_t base = tile * F::kTile;
+  const uint32_t tile_n = min(static_cast<uint32_t>(F::kTile), n - base);
+  for (uint32_t e = t; e < nb; e += T) s_cnt[e] = 0u;
   __syncthreads();
   uint32_t dst[kLabelItems], rk[kLabelItems];
   int32_t lab[kLabelItems];
 #pragma unroll
   for (int k = 0; k < kLabelItems; ++k) {
-    const uint32_t l = k * kLabelThreads + t;
+    const uint32_t l = k * T + t;
     dst[k] = n;
     if (l < tile_n) {
       const uint32_t i = base + l;
       dst[k] = idx[i];
       lab[k] = is_core[i] ? cell_cluster[point_cell[i]] : lab_sorted[i];
-      if (dst[k] < n) rk[k] = atomicAdd(&s_cnt[dst[k] >> S], 1u);
+      if (dst[k] < n) rk[k] = atomicAdd(&s_cnt[dst[k] / B], 1u);
     }
   }
   __syncthreads();
   const bool owner = static_cast<uint32_t>(t) < nb;
   const uint32_t cnt = owner ? s_cnt[t] : 0u;
   uint32_t ttot;
-  const uint32_t lstart = block_excl_scan<kLabelThreads>(cnt, ttot, s_scan);
+  const uint32_t lstart = block_excl_scan<T>(cnt, ttot, s_scan);
   if (owner) {
     uint64_t excl = 0;
     st_release(status + static_cast<uint64_t>(tile) * kMaxBuckets + t,
@@ -2173,34 +2194,46 @@ __global__ __launch_bounds__(kLabelThreads) void k_label(
       st_release(status + static_cast<uint64_t>(tile) * kMaxBuckets + t,
                  kLabelTagPre | (excl + cnt));
     }
-    s_base[t] = (static_cast<uint32_t>(t) << S) + static_cast<uint32_t>(excl) - lstart;
+    s_base[t] = static_cast<uint32_t>(t) * B + static_cast<uint32_t>(excl);
+    s_len[t] = cnt;
     s_cnt[t] = lstart;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kLabelItems; ++k) {
     if (dst[k] < n) {
-      s_buf[s_cnt[dst[k] >> S] + rk[k]] =
-          (static_cast<uint64_t>(static_cast<uint32_t>(lab[k])) << 32) | dst[k];
-    } else if (k * kLabelThreads + t < tile_n) {
+      const uint32_t b = dst[k] / B;
+      if constexpr (kPacked)
+        s_buf[s_cnt[b] + rk[k]] = (static_cast<uint32_t>(lab[k] + 1) << 15) | (dst[k] - b * B);
+      else
+        s_buf[s_cnt[b] + rk[k]] =
+            (static_cast<uint64_t>(static_cast<uint32_t>(lab[k])) << 32) | dst[k];
+    } else if (k * T + t < tile_n) {
       atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
     }
   }
   __syncthreads();
-  for (uint32_t j = t; j < tile_n; j += kLabelThreads) {
-    const uint64_t v = s_buf[j];
-    const uint32_t d = static_cast<uint32_t>(v), b = d >> S;
-    const uint32_t pos = s_base[b] + j;
-    if (pos < n && pos < ((b + 1u) << S)) {
-      pair_i[pos] = d;
-      pair_l[pos] = static_cast<int32_t>(v >> 32);
-    } else {
-      atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+  // each run by a group of 16 lanes
+  const uint32_t g = static_cast<uint32_t>(t) >> 4, gl = static_cast<uint32_t>(t) & 15u;
+  for (uint32_t b = g; b < nb; b += T / 16) {
+    const uint32_t len = s_len[b], ls = s_cnt[b], gs = s_base[b];
+    if (gs + len > min(n, (b + 1u) * B)) {
+      if (gl == 0) atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+      continue;
+    }
+    for (uint32_t k = gl; k < len; k += 16) {
+      const typename F::Slot v = s_buf[ls + k];
+      if constexpr (kPacked) {
+        pair_i[gs + k] = v;
+      } else {
+        pair_i[gs + k] = static_cast<uint32_t>(v);
+        pair_l[gs + k] = static_cast<int32_t>(v >> 32);
+      }
     }
   }
 }
 
-// Inputs of more than kMaxBuckets << kUnpermLdsBits points: labels written
+// Inputs of more than kMaxBuckets * kUnpermMaxWidth points: labels written
 // straight to their input positions (one 4-byte write per label).
 __global__ __launch_bounds__(256) void k_label_direct(
     const Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
@@ -2219,25 +2252,34 @@ __global__ __launch_bounds__(256) void k_label_direct(
 }
 
 // Pass B: one workgroup per bucket places the bucket's labels in LDS by input
-// position and writes them out in order.
-__global__ __launch_bounds__(kUnpermThreads) void k_unpermute(
+// position and writes them out in order (B <= kWidth).
+template <int kWidth, int kThreads, bool kPacked>
+__global__ __launch_bounds__(kThreads) void k_unpermute(
     const Params* __restrict__ prm, const uint32_t* __restrict__ spare_xa,
     const uint32_t* __restrict__ spare_xb, const uint32_t* __restrict__ spare_ya,
-    const uint32_t* __restrict__ spare_yb, int32_t* __restrict__ labels, int64_t n64, int S) {
-  __shared__ int32_t s_lab[1 << kUnpermLdsBits];
+    const uint32_t* __restrict__ spare_yb, int32_t* __restrict__ labels, int64_t n64,
+    uint32_t B) {
+  __shared__ int32_t s_lab[kWidth];
+  if ((prm->num_roots < kPackedRoots) != kPacked) return;  // the other form's launch
   const uint32_t n = static_cast<uint32_t>(n64);
   const bool in_a = (prm->npasses & 1) != 0;
   const uint32_t* __restrict__ pair_i = in_a ? spare_xb : spare_xa;
   const int32_t* __restrict__ pair_l = reinterpret_cast<const int32_t*>(in_a ? spare_yb : spare_ya);
   const int t = threadIdx.x;
-  const uint32_t b0 = static_cast<uint32_t>(blockIdx.x) << S;  // S == kUnpermLdsBits
-  const uint32_t b1 = min(n, b0 + (1u << S));
-  for (uint32_t j = b0 + t; j < b1; j += kUnpermThreads) {
-    const uint32_t d = pair_i[j];
-    if (d >= b0 && d < b1) s_lab[d - b0] = pair_l[j];
+  const uint32_t b0 = static_cast<uint32_t>(blockIdx.x) * B;
+  const uint32_t b1 = min(n, b0 + min(B, static_cast<uint32_t>(kWidth)));
+  for (uint32_t j = b0 + t; j < b1; j += kThreads) {
+    if constexpr (kPacked) {
+      const uint32_t v = pair_i[j];
+      const uint32_t d = b0 + (v & 0x7FFFu);
+      if (d < b1) s_lab[d - b0] = static_cast<int32_t>(v >> 15) - 1;
+    } else {
+      const uint32_t d = pair_i[j];
+      if (d >= b0 && d < b1) s_lab[d - b0] = pair_l[j];
+    }
   }
   __syncthreads();
-  for (uint32_t j = b0 + t; j < b1; j += kUnpermThreads) labels[j] = s_lab[j - b0];
+  for (uint32_t j = b0 + t; j < b1; j += kThreads) labels[j] = s_lab[j - b0];
 }
 
 // Border search for one point by one lane (dbscan_grid.hpp:419-445): window
@@ -2474,13 +2516,47 @@ const char* const kStageNames[DBSCAN_HIP_NUM_STAGES] = {
     "cell_keys", "sort_gather", "boundary_scan", "build_ngh", "core_mark",
     "union_find", "cluster_id", "core_labels", "border_assign", "unpermute"};
 
-// Bucket width S of the label unpermute: 2^kUnpermLdsBits input positions
-// per bucket (one LDS load of k_unpermute), or 0 when that needs more than
-// kMaxBuckets buckets (k_label_direct then writes the labels).
-int bucket_bits(int64_t n) {
-  return ((n + (int64_t(1) << kUnpermLdsBits) - 1) >> kUnpermLdsBits) <= kMaxBuckets
-             ? kUnpermLdsBits
-             : 0;
+// Bucket width B of the label unpermute (input positions per bucket), or 0
+// when more than kMaxBuckets buckets of kUnpermMaxWidth would be needed
+// (k_label_direct then writes the labels).
+uint32_t bucket_width(int64_t n) {
+  int64_t B = (n + kMaxBuckets - 1) / kMaxBuckets;
+  B = std::max<int64_t>(4096, (B + 255) / 256 * 256);
+  return B <= kUnpermMaxWidth ? static_cast<uint32_t>(B) : 0u;
+}
+
+void launch_unpermute(uint32_t B, int64_t n, hipStream_t stream, const Params* prm,
+                      float* xs_a, float* xs_b, float* ys_a, float* ys_b, int32_t* labels) {
+  const int nb = static_cast<int>((n + B - 1) / B);
+  auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
+  if (B <= static_cast<uint32_t>(kUnpermSmallWidth)) {
+    k_unpermute<kUnpermSmallWidth, 512, true><<<nb, 512, 0, stream>>>(
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
+    k_unpermute<kUnpermSmallWidth, 512, false><<<nb, 512, 0, stream>>>(
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
+  } else {
+    k_unpermute<kUnpermMaxWidth, 1024, true><<<nb, 1024, 0, stream>>>(
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
+    k_unpermute<kUnpermMaxWidth, 1024, false><<<nb, 1024, 0, stream>>>(
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
+  }
+}
+
+int grid_for(int64_t items, int per_block, int cap);
+
+// Both forms of the label pass (the one that does not match num_roots
+// returns at once).
+void launch_label(uint32_t B, int64_t n, hipStream_t stream, Params* prm, const uint32_t* idx_a,
+                  const uint32_t* idx_b, const int32_t* point_cell, const uint8_t* is_core,
+                  const int32_t* cell_cluster, float* xs_a, float* xs_b, float* ys_a,
+                  float* ys_b, uint64_t* status) {
+  auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
+  k_label<true><<<grid_for(n, LabelForm<true>::kTile, INT_MAX), LabelForm<true>::kThreads, 0,
+                  stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
+                            u(xs_a), u(xs_b), u(ys_a), u(ys_b), status, n, B);
+  k_label<false><<<grid_for(n, LabelForm<false>::kTile, INT_MAX), LabelForm<false>::kThreads, 0,
+                   stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
+                             u(xs_a), u(xs_b), u(ys_a), u(ys_b), status, n, B);
 }
 
 int grid_for(int64_t items, int per_block, int cap) {
@@ -2756,23 +2832,16 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
                                             ngh, cell_flags, cell_box, is_core, cell_cluster, pp);
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  const int S = bucket_bits(n);
-  if (S > 0) {
-    k_label<<<grid_for(n, kLabelTile, INT_MAX), kLabelThreads, 0, stream>>>(
-        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
-        reinterpret_cast<uint32_t*>(xs_a), reinterpret_cast<uint32_t*>(xs_b),
-        reinterpret_cast<uint32_t*>(ys_a), reinterpret_cast<uint32_t*>(ys_b), sort_status, n, S);
+  const uint32_t bw = bucket_width(n);
+  if (bw > 0) {
+    launch_label(bw, n, stream, prm, idx_a, idx_b, point_cell, is_core, cell_cluster, xs_a, xs_b,
+                 ys_a, ys_b, sort_status);
   } else {
     k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
         prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
   }
   if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
-  if (S > 0) {
-    k_unpermute<<<static_cast<int>((n + (int64_t(1) << S) - 1) >> S), kUnpermThreads, 0,
-                  stream>>>(prm, reinterpret_cast<uint32_t*>(xs_a),
-                            reinterpret_cast<uint32_t*>(xs_b), reinterpret_cast<uint32_t*>(ys_a),
-                            reinterpret_cast<uint32_t*>(ys_b), labels, n, S);
-  }
+  if (bw > 0) launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, labels);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
@@ -2974,23 +3043,16 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
                                             ngh, cell_flags, cell_box, is_core, cell_cluster, pp);
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  const int S = bucket_bits(n);
-  if (S > 0) {
-    k_label<<<grid_for(n, kLabelTile, INT_MAX), kLabelThreads, 0, stream>>>(
-        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
-        reinterpret_cast<uint32_t*>(xs_a), reinterpret_cast<uint32_t*>(xs_b),
-        reinterpret_cast<uint32_t*>(ys_a), reinterpret_cast<uint32_t*>(ys_b), sort_status, n, S);
+  const uint32_t bw = bucket_width(n);
+  if (bw > 0) {
+    launch_label(bw, n, stream, prm, idx_a, idx_b, point_cell, is_core, cell_cluster, xs_a, xs_b,
+                 ys_a, ys_b, sort_status);
   } else {
     k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
         prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
   }
   if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
-  if (S > 0) {
-    k_unpermute<<<static_cast<int>((n + (int64_t(1) << S) - 1) >> S), kUnpermThreads, 0,
-                  stream>>>(prm, reinterpret_cast<uint32_t*>(xs_a),
-                            reinterpret_cast<uint32_t*>(xs_b), reinterpret_cast<uint32_t*>(ys_a),
-                            reinterpret_cast<uint32_t*>(ys_b), labels, n, S);
-  }
+  if (bw > 0) launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, labels);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   return DBSCAN_HIP_OK;

@@ -118,6 +118,19 @@ def test_twenty_million_points(dt):
     assert np.array_equal(labels, oracle.dbscan2d(X2, eps, ms))
 
 
+@pytest.mark.parametrize("clusters", [131_070, 131_071, 300_000])
+def test_label_pair_forms(dt, clusters):
+    """Isolated points with min_samples 1 are one cluster each.  Below
+    131,071 clusters a label pair travels as one packed word, from there on as
+    two words; both sides of the switch against the oracle."""
+    side = int(np.ceil(np.sqrt(clusters)))
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)[:clusters]
+    X = g[np.random.default_rng(clusters).permutation(clusters)].astype(np.float32)
+    labels = gpu_labels(dt, X, 0.5, 1)
+    assert int(labels.max()) + 1 == clusters
+    assert np.array_equal(labels, oracle.dbscan2d(X, 0.5, 1))
+
+
 def test_permutation_equivariance_lattice1m(dt):
     X, eps, ms = W.lattice1m()
     base = gpu_labels(dt, X, eps, ms)
