@@ -363,6 +363,7 @@ __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ out, int
     out[i] = v;
 }
 
+template <int kWaves = 4>
 __device__ __forceinline__ void block_minmax4(int32_t v[4], int32_t (*s)[4]) {
   // v = {min, max, min, max}
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -380,7 +381,7 @@ __device__ __forceinline__ void block_minmax4(int32_t v[4], int32_t (*s)[4]) {
   __syncthreads();
   v[0] = s[0][0]; v[1] = s[0][1]; v[2] = s[0][2]; v[3] = s[0][3];
 #pragma unroll
-  for (int i = 1; i < 4; ++i) {
+  for (int i = 1; i < kWaves; ++i) {
     v[0] = min(v[0], s[i][0]);
     v[1] = max(v[1], s[i][1]);
     v[2] = min(v[2], s[i][2]);
@@ -427,21 +428,27 @@ __global__ __launch_bounds__(256) void k_bounds(const float2* __restrict__ X, in
 }
 
 // Reduces the bound partials, publishes them, and accumulates the digit
-// histogram of every radix pass.
-__global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int64_t n,
+// histogram of every radix pass.  One large workgroup per CU: each adds its
+// histogram into the global one at the end, and every such atomic lands on
+// one of only npasses x 128 words (512 workgroups of 256: 36 us for 10M
+// points; 256 of 1,024: 30 us; 128 or fewer leave CUs idle).
+constexpr int kHistThreads = 1024;
+constexpr int kHistBlocks = 256;
+__global__ __launch_bounds__(kHistThreads) void k_hist(const float2* __restrict__ X, int64_t n,
                                               float inv_r, const int4* __restrict__ partials,
                                               int nparts, Params* __restrict__ prm,
                                               int2* __restrict__ col_range, int64_t max_cols,
                                               int rbits) {
-  __shared__ int32_t s_red[4][4];
+  constexpr int T = kHistThreads;
+  __shared__ int32_t s_red[T / 64][4];
   __shared__ uint32_t s_hist[kMaxPasses][kRadixMax];
   int32_t v[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-  for (int i = threadIdx.x; i < nparts; i += 256) {
+  for (int i = threadIdx.x; i < nparts; i += T) {
     const int4 q = partials[i];
     v[0] = min(v[0], q.x); v[1] = max(v[1], q.y);
     v[2] = min(v[2], q.z); v[3] = max(v[3], q.w);
   }
-  block_minmax4(v, s_red);
+  block_minmax4<T / 64>(v, s_red);
   const KeyMap m = make_keymap(v[0], v[1], v[2], v[3], 0, rbits);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     prm->kxmin = v[0]; prm->kxmax = v[1]; prm->kymin = v[2]; prm->kymax = v[3];
@@ -450,10 +457,10 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
     prm->error = 0;
     prm->col_total = 0;
   }
-  for (int e = threadIdx.x; e < kMaxPasses * kRadixMax; e += 256) (&s_hist[0][0])[e] = 0u;
+  for (int e = threadIdx.x; e < kMaxPasses * kRadixMax; e += T) (&s_hist[0][0])[e] = 0u;
   __syncthreads();
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
-  const int64_t gtid = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * T;
+  const int64_t gtid = static_cast<int64_t>(blockIdx.x) * T + threadIdx.x;
   if (has_col_index(m, max_cols))
     for (int64_t q = gtid; q < (1ll << m.bx); q += stride) col_range[q] = make_int2(0, 0);
   auto take = [&](float x, float y) {
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(256) void k_hist(const float2* __restrict__ X, int6
     if ((n & 1) && gtid == 0) take(X[n - 1].x, X[n - 1].y);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < m.npasses * kRadixMax; e += 256) {
+  for (int e = threadIdx.x; e < m.npasses * kRadixMax; e += T) {
     const uint32_t c = (&s_hist[0][0])[e];
     if (c) atomicAdd(&prm->hist[0][0] + e, c);
   }
@@ -1124,25 +1131,55 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
     const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
     const uint32_t yhi = yr <= 0xFFFFFFFDu ? yr + 2u : 0xFFFFFFFFu;
     int32_t* row = ngh + c64 * 10;
-    int32_t own_lo = 0, own_hi = 0;
+    // The five columns' searches run in lock-step, so that each step is one
+    // round of five independent loads rather than five dependent ones.
+    int32_t lo[5], hi[5], base[5], len[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const int64_t cx = static_cast<int64_t>(xr) + (k - 2);
-      int32_t j0 = 0, j1 = 0;
-      if (cx >= 0 && cx <= 0xFFFFFFFFll) {
-        int32_t lo, hi;
-        column(cx, lo, hi);
-        if (k == 2) {
-          own_lo = lo;
-          own_hi = hi;
+      lo[k] = hi[k] = 0;
+      if (cx >= 0 && cx <= 0xFFFFFFFFll) column(cx, lo[k], hi[k]);
+      base[k] = lo[k];
+      len[k] = hi[k] - lo[k];
+    }
+    const int32_t own_lo = lo[2], own_hi = hi[2];
+    // lower bound of (column, ylo): first cell of the column at y' >= ylo
+    // (all five loads of a step issued before any is used: a finished or
+    // empty search loads cell c, which exists)
+    for (uint32_t it = 0; it < 40; ++it) {  // len < 2^31: at most 32 halvings
+      uint64_t v[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) v[k] = cell_key[len[k] > 0 ? base[k] + (len[k] >> 1) : c];
+      bool more = false;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const uint64_t colbase = static_cast<uint64_t>(static_cast<int64_t>(xr) + (k - 2)) << 32;
+        const int32_t half = len[k] >> 1;
+        const bool right = v[k] < (colbase | ylo);
+        if (len[k] > 0) {
+          base[k] = right ? base[k] + half + 1 : base[k];
+          len[k] = right ? len[k] - half - 1 : half;
         }
-        const uint64_t colbase = static_cast<uint64_t>(cx) << 32;
-        j0 = lo + lower_bound_u64(cell_key + lo, hi - lo, colbase | ylo);
-        j1 = j0;
-        while (j1 < hi && cell_key[j1] <= (colbase | yhi)) ++j1;
+        more |= len[k] > 0;
       }
-      row[2 * k] = j0;
-      row[2 * k + 1] = j1;
+      if (!more) break;
+    }
+    // the column's cells up to y' <= yhi: at most 5 (one per y'), keys
+    // ascending, so the qualifying ones are a prefix
+    uint64_t w[5][5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) w[k][q] = cell_key[base[k] + q < hi[k] ? base[k] + q : c];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const uint64_t lim =
+          (static_cast<uint64_t>(static_cast<int64_t>(xr) + (k - 2)) << 32) | yhi;
+      int32_t cnt = 0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) cnt += (base[k] + q < hi[k] && w[k][q] <= lim) ? 1 : 0;
+      row[2 * k] = base[k];
+      row[2 * k + 1] = base[k] + cnt;
     }
     int32_t r = c;
     if (straddle) {
@@ -2749,7 +2786,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   int32_t* cell_cluster = at<int32_t>(ws, L.cell_cluster);
 
   const int g_reduce = grid_for(n, 2048, kMaxReduceBlocks);
-  const int g_hist = grid_for(n, 4096, kMaxReduceBlocks);
+  const int g_hist = grid_for(n, 16 * kHistThreads, kHistBlocks);
   const int g_sort = grid_for(n, kSortThreads * cfg.sort_items, INT_MAX);
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
@@ -2763,7 +2800,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   k_bounds<<<g_reduce, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials),
                                          at<uint4>(ws, kParamsZeroBegin), zero16);
   if ((e = before(kKHist)) != hipSuccess) return fail(e);
-  k_hist<<<g_hist, 256, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm,
+  k_hist<<<g_hist, kHistThreads, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm,
                                      at<int2>(ws, L.col_range), L.max_cols, cfg.radix_bits);
 
   // Stage 2: radix passes (sort + gather in one).
