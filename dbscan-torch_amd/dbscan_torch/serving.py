@@ -64,8 +64,10 @@ class Request:
 
 
 class _Slot:
-    """One batch's share of the arena: points, labels, workspace, its stream
-    and a pinned host buffer for the labels."""
+    """One batch's share of the arena: points, labels, workspace and its
+    stream.  Each request's labels go device-to-host straight into a pinned
+    tensor of its own (torch's caching host allocator), so retiring a batch
+    copies nothing on the host."""
 
     def __init__(self, arena: torch.Tensor, offset: int, points: int, ws_bytes: int,
                  device: torch.device):
@@ -74,10 +76,9 @@ class _Slot:
         self.labels = arena.narrow(0, offset + x_bytes, points * 4).view(torch.int32)
         self.workspace = arena.narrow(0, offset + x_bytes + l_bytes, ws_bytes)
         self.stream = torch.cuda.Stream(device)  # default priority
-        self.host_labels = torch.empty(points, dtype=torch.int32, pin_memory=True)
         self.event: Optional[torch.cuda.Event] = None
         self.batch: List[Request] = []
-        self.offsets: List[int] = []
+        self.host_labels: List[torch.Tensor] = []
 
     @staticmethod
     def nbytes(points: int, ws_bytes: int) -> int:
@@ -157,6 +158,7 @@ class Server:
             pts += r.n
             offs.append(pts)
         now = self.clock()
+        host = [torch.empty(r.n, dtype=torch.int32, pin_memory=True) for r in batch]
         with torch.cuda.stream(slot.stream):
             for r, o in zip(batch, offs):
                 r.admitted = now
@@ -164,20 +166,21 @@ class Server:
             _C.dbscan2d_batched_out(slot.X[:pts], offs, [r.eps for r in batch],
                                     [r.min_samples for r in batch], slot.labels,
                                     slot.workspace)
-            slot.host_labels[:pts].copy_(slot.labels[:pts], non_blocking=True)
+            for h, r, o in zip(host, batch, offs):
+                h.copy_(slot.labels[o:o + r.n], non_blocking=True)
             slot.event = torch.cuda.Event()
             slot.event.record(slot.stream)
-        slot.batch, slot.offsets = batch, offs
+        slot.batch, slot.host_labels = batch, host
         self.inflight.append(slot)
         self.batches.append(len(batch))
 
     def _retire(self, slot: _Slot) -> None:
         now = self.clock()
-        for r, o in zip(slot.batch, slot.offsets):
-            r.labels = slot.host_labels[o:o + r.n].clone()
+        for r, h in zip(slot.batch, slot.host_labels):
+            r.labels = h
             r.done = now
             self.completed[r.rid] = r
-        slot.batch, slot.offsets, slot.event = [], [], None
+        slot.batch, slot.host_labels, slot.event = [], [], None
         self.free.append(slot)
 
     def step(self) -> bool:
