@@ -100,11 +100,14 @@ struct Params {
   int32_t error;
   int32_t col_total;  // batches: column-index entries in use
   int32_t num_roots;  // roots over all cells (k_rank): bounds every cluster id
-  // Everything below is cleared by k_bounds at the start of every call.
+  int32_t pad_[3];
+  // Everything below is cleared by k_bounds at the start of every call, in
+  // 16-byte stores from here to the end of the look-back status words.
   uint32_t tile_counter[16];
   uint32_t hist[kMaxPasses][kRadixMax];
 };
 constexpr size_t kParamsZeroBegin = offsetof(Params, tile_counter);
+static_assert(kParamsZeroBegin % 16 == 0, "the cleared range starts 16-byte aligned");
 
 // ---------------------------------------------------------------------------
 // Device helpers
@@ -1241,21 +1244,62 @@ __device__ __forceinline__ float bcastf(float v, int src) {
 // Per-cell bounding boxes of the non-NaN points (the reference's per-cell
 // boxes, dbscan_grid.hpp:332-349, used only for sound prunes), and core
 // marking pass 1 (dbscan_grid.hpp:270-279): every point of a cell with at
-// least min_samples points is core.  Each lane holds one point of each of
-// kBoxChunks consecutive runs of 64 sorted points; the runs of equal cells
-// in a chunk are reduced with shuffles.  A cell that lies wholly inside a
-// chunk is stored directly; a run touching either end of the chunk is folded,
-// with atomics, into the sentinel written by k_cells, the runs covering whole
-// chunks first merged per block in LDS (a cell of 300k points then costs 4
-// atomics per 1024 points, not per 64).
+// least min_samples points is core.  Each thread takes kBoxItems consecutive
+// sorted points and reduces their runs of equal cells in registers; a run
+// wholly inside the thread is a whole cell, stored directly.  The runs that
+// touch a thread's ends are joined across the wave by a segmented suffix
+// reduction over the threads' head runs (shuffles, once per thread rather
+// than per point); a cell that stays inside the wave is stored directly, one
+// reaching past it waits in the wave's LDS slot, and the slots of the
+// workgroup are merged before they are folded, with atomics, into the
+// sentinel written by k_cells (a cell of 300k points then costs 4 atomics per
+// 2,048 points).
 // cell_flags bits: kCellHasNaN and, for full cells, kCellHasCore are set by
 // k_bbox; k_core_sparse sets the core bits of the other cells.
 constexpr uint32_t kCellHasCore = 1u, kCellHasNonCore = 2u, kCellHasNaN = 4u;
 
-// Each wave takes kBoxChunks consecutive runs of 64 points (lane = point in
-// each), with all their loads issued before the first reduction.
-constexpr int kBoxChunks = 4;
-constexpr int kBoxPointsPerBlock = 256 * kBoxChunks;
+constexpr int kBoxItems = 8;
+constexpr int kBoxTile = 256 * kBoxItems;
+
+// A run's box (ordered-integer bounds) and flag bits.
+struct BoxPiece {
+  uint32_t mnx, mxx, mny, mxy, fl;
+};
+__device__ __forceinline__ BoxPiece piece_empty() {
+  return BoxPiece{0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0u};
+}
+__device__ __forceinline__ void piece_add(BoxPiece& a, const BoxPiece& b) {
+  a.mnx = min(a.mnx, b.mnx);
+  a.mxx = max(a.mxx, b.mxx);
+  a.mny = min(a.mny, b.mny);
+  a.mxy = max(a.mxy, b.mxy);
+  a.fl |= b.fl;
+}
+__device__ __forceinline__ BoxPiece piece_shfl_down(const BoxPiece& p, int d) {
+  auto f = [&](uint32_t v) {
+    return static_cast<uint32_t>(__shfl_down(static_cast<int>(v), d, 64));
+  };
+  return BoxPiece{f(p.mnx), f(p.mxx), f(p.mny), f(p.mxy), f(p.fl)};
+}
+// A whole cell: its box stored over the sentinel.
+__device__ __forceinline__ void piece_store(uint4* __restrict__ cell_box,
+                                            uint32_t* __restrict__ cell_flags, int32_t c,
+                                            const BoxPiece& p) {
+  cell_box[c] = make_uint4(p.mnx, p.mxx, p.mny, p.mxy);
+  if (p.fl) atomicOr(cell_flags + c, p.fl);
+}
+// Part of a cell: folded in with atomics.
+__device__ __forceinline__ void piece_fold(uint4* __restrict__ cell_box,
+                                           uint32_t* __restrict__ cell_flags, int32_t c,
+                                           const BoxPiece& p) {
+  if (p.mnx != 0xFFFFFFFFu) {
+    atomicMin(&cell_box[c].x, p.mnx);
+    atomicMax(&cell_box[c].y, p.mxx);
+    atomicMin(&cell_box[c].z, p.mny);
+    atomicMax(&cell_box[c].w, p.mxy);
+  }
+  if (p.fl) atomicOr(cell_flags + c, p.fl);
+}
 
 template <class PP>
 __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
@@ -1269,131 +1313,150 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
                                               uint32_t* __restrict__ cell_flags,
                                               uint8_t* __restrict__ is_core, int64_t n,
                                               PP pp) {
-  __shared__ int32_t s_pc[4 * kBoxChunks];  // whole-chunk runs: cell (or -1)
-  __shared__ uint4 s_pb[4 * kBoxChunks];     // and their boxes
-  __shared__ uint32_t s_pf[4 * kBoxChunks];  // and flag bits
-  bool any_whole = false;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // per wave: [2w] the cell continuing in from before the wave, [2w + 1] the
+  // cell continuing past it (-1: none)
+  __shared__ int32_t s_key[8];
+  __shared__ BoxPiece s_val[8];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  const int64_t wbase = static_cast<int64_t>(blockIdx.x) * kBoxPointsPerBlock +
-                        static_cast<int64_t>(wv) * (64 * kBoxChunks);
-  int32_t cc[kBoxChunks];
-  float px[kBoxChunks], py[kBoxChunks];
-  bool full[kBoxChunks];
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kBoxTile +
+                     static_cast<int64_t>(t) * kBoxItems;
+  int32_t pc[kBoxItems];
+  float px[kBoxItems], py[kBoxItems];
+  if (e0 + kBoxItems <= n) {
+    const int4 c0 = *reinterpret_cast<const int4*>(point_cell + e0);
+    const int4 c1 = *reinterpret_cast<const int4*>(point_cell + e0 + 4);
+    const float4 x0 = *reinterpret_cast<const float4*>(xs + e0);
+    const float4 x1 = *reinterpret_cast<const float4*>(xs + e0 + 4);
+    const float4 y0 = *reinterpret_cast<const float4*>(ys + e0);
+    const float4 y1 = *reinterpret_cast<const float4*>(ys + e0 + 4);
+    pc[0] = c0.x; pc[1] = c0.y; pc[2] = c0.z; pc[3] = c0.w;
+    pc[4] = c1.x; pc[5] = c1.y; pc[6] = c1.z; pc[7] = c1.w;
+    px[0] = x0.x; px[1] = x0.y; px[2] = x0.z; px[3] = x0.w;
+    px[4] = x1.x; px[5] = x1.y; px[6] = x1.z; px[7] = x1.w;
+    py[0] = y0.x; py[1] = y0.y; py[2] = y0.z; py[3] = y0.w;
+    py[4] = y1.x; py[5] = y1.y; py[6] = y1.z; py[7] = y1.w;
+  } else {
 #pragma unroll
-  for (int k = 0; k < kBoxChunks; ++k) {
-    const int64_t i = wbase + k * 64 + lane;
-    const int64_t ic = i < n ? i : n - 1;  // clamped: defined, never used
-    cc[k] = point_cell[ic];
-    px[k] = xs[ic];
-    py[k] = ys[ic];
-  }
-#pragma unroll
-  for (int k = 0; k < kBoxChunks; ++k) {
-    const int64_t i = wbase + k * 64 + lane;
-    full[k] = cell_start[cc[k] + 1] - cell_start[cc[k]] >= pp.at(cc[k]).ms;
-    if (i < n) is_core[i] = full[k] ? 1 : 0;  // other cells' points: k_core_sparse
-  }
-#pragma unroll
-  for (int k = 0; k < kBoxChunks; ++k) {
-    const int64_t i = wbase + k * 64 + lane;
-    const bool valid = i < n;
-    const int32_t c = valid ? cc[k] : -1;
-    uint32_t mnx = 0xFFFFFFFFu, mxx = 0u, mny = 0xFFFFFFFFu, mxy = 0u;
-    int32_t has_nan = 0;
-    if (valid) {
-      if (!isnan(px[k]) && !isnan(py[k])) {
-        mnx = mxx = f2ord(px[k]);
-        mny = mxy = f2ord(py[k]);
-      } else {
-        has_nan = 1;
-      }
-    }
-    // Suffix reduction within runs of equal c (runs are contiguous: sorted).
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int32_t oc = __shfl_down(c, d, 64);
-      const uint32_t a0 = __shfl_down(mnx, d, 64), a1 = __shfl_down(mxx, d, 64);
-      const uint32_t a2 = __shfl_down(mny, d, 64), a3 = __shfl_down(mxy, d, 64);
-      const int32_t a4 = __shfl_down(has_nan, d, 64);
-      if (lane + d < 64 && oc == c) {
-        mnx = min(mnx, a0); mxx = max(mxx, a1);
-        mny = min(mny, a2); mxy = max(mxy, a3);
-        has_nan |= a4;
-      }
-    }
-    const int32_t prev_c = __shfl_up(c, 1, 64);
-    const bool head = valid && (lane == 0 || prev_c != c);
-    // Lane of the last point of each run: the run ends inside the chunk if
-    // the next lane holds another cell (or is past n).
-    const int32_t next_c = __shfl_down(c, 1, 64);
-    const bool tail = valid && (lane == 63 || next_c != c);
-    const uint64_t tails = __ballot(tail);
-    const uint32_t fl = (has_nan ? kCellHasNaN : 0u) | (full[k] ? kCellHasCore : 0u);
-    // A run wholly inside the chunk is stored; one touching an end of the
-    // chunk is folded in with atomics, except a run covering the whole chunk
-    // (a cell of 64 points or more), which waits in the chunk's LDS slot for
-    // the block's merge below.
-    const int slot = wv * kBoxChunks + k;
-    if (lane == 0) s_pc[slot] = -1;
-    if (head) {
-      // the run's last lane: first tail at or after this lane
-      const uint64_t after = tails & ~((1ull << lane) - 1ull);
-      const int last = lowest_lane(after);
-      const bool starts_here = lane > 0 || i == 0;
-      const bool ends_here = last < 63 || i + (last - lane) == n - 1;
-      if (starts_here && ends_here) {
-        cell_box[c] = make_uint4(mnx, mxx, mny, mxy);
-        if (fl) atomicOr(cell_flags + c, fl);
-      } else if (lane == 0 && last == 63) {
-        s_pc[slot] = c;
-        s_pb[slot] = make_uint4(mnx, mxx, mny, mxy);
-        s_pf[slot] = fl;
-        any_whole = true;
-      } else {
-        if (mnx != 0xFFFFFFFFu) {
-          atomicMin(&cell_box[c].x, mnx);
-          atomicMax(&cell_box[c].y, mxx);
-          atomicMin(&cell_box[c].z, mny);
-          atomicMax(&cell_box[c].w, mxy);
-        }
-        if (fl) atomicOr(cell_flags + c, fl);
-      }
+    for (int j = 0; j < kBoxItems; ++j) {
+      const bool v = e0 + j < n;
+      pc[j] = v ? point_cell[e0 + j] : -1;  // -1: past n, never stored
+      px[j] = v ? xs[e0 + j] : 0.f;
+      py[j] = v ? ys[e0 + j] : 0.f;
     }
   }
-  // Whole-chunk runs of the block, in point order: equal neighbours are one
-  // cell, folded in once.  (Such a cell always reaches past these chunks.)
-  if (__syncthreads_or(any_whole) && threadIdx.x == 0) {
+  // full cells: every point core (the loads of all items issued together)
+  bool full[kBoxItems];
+#pragma unroll
+  for (int j = 0; j < kBoxItems; ++j) {
+    const int32_t c = pc[j] >= 0 ? pc[j] : 0;
+    full[j] = pc[j] >= 0 && cell_start[c + 1] - cell_start[c] >= pp.at(c).ms;
+  }
+  if (e0 + kBoxItems <= n) {
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      lo |= static_cast<uint32_t>(full[j]) << (8 * j);
+      hi |= static_cast<uint32_t>(full[j + 4]) << (8 * j);
+    }
+    *reinterpret_cast<uint2*>(is_core + e0) = make_uint2(lo, hi);  // other cells: k_core_sparse
+  } else {
+#pragma unroll
+    for (int j = 0; j < kBoxItems; ++j)
+      if (e0 + j < n) is_core[e0 + j] = full[j] ? 1 : 0;
+  }
+  auto item = [&](int j) {
+    BoxPiece p = piece_empty();
+    if (!isnan(px[j]) && !isnan(py[j])) {
+      p.mnx = p.mxx = f2ord(px[j]);
+      p.mny = p.mxy = f2ord(py[j]);
+    } else {
+      p.fl = kCellHasNaN;
+    }
+    if (full[j]) p.fl |= kCellHasCore;
+    return p;
+  };
+  // the thread's runs: head (holds item 0), tail (holds the last item); runs
+  // strictly between them are whole cells
+  BoxPiece head = piece_empty(), cur = item(0);
+  int32_t curc = pc[0];
+  int run_start = 0;
+#pragma unroll
+  for (int j = 1; j < kBoxItems; ++j) {
+    if (pc[j] == curc) {
+      piece_add(cur, item(j));
+    } else {
+      if (run_start == 0) head = cur;
+      else if (curc >= 0) piece_store(cell_box, cell_flags, curc, cur);
+      curc = pc[j];
+      cur = item(j);
+      run_start = j;
+    }
+  }
+  const BoxPiece tail = cur;
+  const bool single = run_start == 0;
+  if (single) head = cur;
+  const int32_t hc = pc[0], tc = pc[kBoxItems - 1];
+  int32_t prev_tc = __shfl_up(tc, 1, 64), next_hc = __shfl_down(hc, 1, 64);
+  if (lane == 0) prev_tc = (e0 > 0 && e0 - 1 < n) ? point_cell[e0 - 1] : -1;
+  if (lane == 63) next_hc = (e0 + kBoxItems < n) ? point_cell[e0 + kBoxItems] : -1;
+  const bool link_in = hc >= 0 && prev_tc == hc;
+  const bool link_out = tc >= 0 && next_hc == tc;
+  // whole cells among the end runs
+  if (single) {
+    if (!link_in && !link_out && hc >= 0) piece_store(cell_box, cell_flags, hc, head);
+  } else {
+    if (!link_in && hc >= 0) piece_store(cell_box, cell_flags, hc, head);
+    if (!link_out && tc >= 0) piece_store(cell_box, cell_flags, tc, tail);
+  }
+  // S = head runs reduced over the following threads of the same cell (a
+  // thread's head cell recurs in the next thread only if the thread is one run)
+  BoxPiece S = head;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t o = __shfl_down(hc, d, 64);
+    const BoxPiece q = piece_shfl_down(S, d);
+    if (lane + d < 64 && o == hc) piece_add(S, q);
+  }
+  // the cell that runs out of the wave through lane 63 (-2: none)
+  const int32_t pass63 = bcast(single && link_out ? hc : -2, 63);
+  const BoxPiece S1 = piece_shfl_down(S, 1);
+  if (lane == 0) s_key[2 * wv] = s_key[2 * wv + 1] = -1;
+  __syncthreads();
+  // a cell that starts in this thread's tail run and continues into the next
+  if (link_out && (!single || !link_in)) {
+    BoxPiece total = tail;
+    if (lane < 63) piece_add(total, S1);
+    if (lane == 63 || tc == pass63) {
+      s_key[2 * wv + 1] = tc;  // one such cell per wave
+      s_val[2 * wv + 1] = total;
+    } else {
+      piece_store(cell_box, cell_flags, tc, total);
+    }
+  }
+  if (lane == 0 && link_in) {  // a cell that started before the wave
+    s_key[2 * wv] = hc;
+    s_val[2 * wv] = S;
+  }
+  // the workgroup's slots in point order; equal neighbours are one cell
+  __syncthreads();
+  if (t == 0) {
     int32_t rc = -1;
-    uint4 rb = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
-    uint32_t rf = 0;
-    auto flush = [&]() {
-      if (rc < 0) return;
-      if (rb.x != 0xFFFFFFFFu) {
-        atomicMin(&cell_box[rc].x, rb.x);
-        atomicMax(&cell_box[rc].y, rb.y);
-        atomicMin(&cell_box[rc].z, rb.z);
-        atomicMax(&cell_box[rc].w, rb.w);
-      }
-      if (rf) atomicOr(cell_flags + rc, rf);
-    };
-    for (int q = 0; q < 4 * kBoxChunks; ++q) {
-      const int32_t qc = s_pc[q];
+    BoxPiece rb = piece_empty();
+    for (int q = 0; q < 8; ++q) {
+      const int32_t qc = s_key[q];
       if (qc < 0) continue;
-      const uint4 qb = s_pb[q];
       if (qc == rc) {
-        rb = make_uint4(min(rb.x, qb.x), max(rb.y, qb.y), min(rb.z, qb.z), max(rb.w, qb.w));
-        rf |= s_pf[q];
+        piece_add(rb, s_val[q]);
         continue;
       }
-      flush();
+      if (rc >= 0) piece_fold(cell_box, cell_flags, rc, rb);
       rc = qc;
-      rb = qb;
-      rf = s_pf[q];
+      rb = s_val[q];
     }
-    flush();
+    if (rc >= 0) piece_fold(cell_box, cell_flags, rc, rb);
   }
 }
 
@@ -2824,7 +2887,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
       cell_key, cell_flags, cell_box, at<int2>(ws, L.col_range), L.max_cols, n, inv_r, nullptr,
       nullptr, nullptr);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
-  k_bbox<<<grid_for(n, kBoxPointsPerBlock, INT_MAX), 256, 0, stream>>>(
+  k_bbox<<<grid_for(n, kBoxTile, INT_MAX), 256, 0, stream>>>(
       prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
                                       cell_box, cell_flags, is_core, n, pp);
 
@@ -3047,7 +3110,7 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
       cell_key, cell_flags, cell_box, col_range, L.max_cols, n, 0.f, seg_scan, clouds,
       cell_cloud);
   if ((e = before(kKBbox)) != hipSuccess) return fail(e);
-  k_bbox<<<grid_for(n, kBoxPointsPerBlock, INT_MAX), 256, 0, stream>>>(
+  k_bbox<<<grid_for(n, kBoxTile, INT_MAX), 256, 0, stream>>>(
       prm, xs_a, ys_a, xs_b, ys_b, point_cell, cell_start,
                                       cell_box, cell_flags, is_core, n, pp);
   if ((e = before(kKNgh)) != hipSuccess) return fail(e);

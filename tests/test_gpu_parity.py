@@ -131,6 +131,24 @@ def test_label_pair_forms(dt, clusters):
     assert np.array_equal(labels, oracle.dbscan2d(X, 0.5, 1))
 
 
+def test_back_to_back_calls_reuse_the_workspace(dt):
+    """Calls that reuse a workspace must not see the previous call's look-back
+    words (a stale inclusive prefix once gave the last scan tile prefix 0):
+    200 calls on one input, interleaved with calls on a larger one, all equal
+    to the oracle."""
+    side = 363
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)[:131_070]
+    X = g[np.random.default_rng(5).permutation(len(g))].astype(np.float32)
+    big = np.concatenate([X, X + np.float32(1000.0), X + np.float32(2000.0)])
+    Xd, bigd = torch.from_numpy(X).to(DEV), torch.from_numpy(big).to(DEV)
+    ref = torch.from_numpy(oracle.dbscan2d(X, 0.5, 1)).to(DEV)
+    print(f"[gpu] n={len(X)} x200, n={len(big)} x20", flush=True)
+    for it in range(200):
+        if it % 10 == 0:
+            dt.dbscan2d(bigd, 0.5, 1)
+        assert torch.equal(dt.dbscan2d(Xd, 0.5, 1), ref), it
+
+
 def test_permutation_equivariance_lattice1m(dt):
     X, eps, ms = W.lattice1m()
     base = gpu_labels(dt, X, eps, ms)
