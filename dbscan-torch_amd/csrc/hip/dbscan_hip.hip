@@ -2212,20 +2212,20 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
 constexpr int kMaxBuckets = 512;
 constexpr int kUnpermMaxWidth = 1 << 15;  // labels k_unpermute holds in LDS at once (128 KB)
 constexpr int kUnpermSmallWidth = 20480;  // up to here the 80 KB instantiation
-constexpr int kLabelItems = 16;
-constexpr int kLabelTile = 512 * kLabelItems;  // the smaller tile of the two forms
+constexpr int kLabelThreads = 1024;
+constexpr int kLabelTile = kLabelThreads * 8;  // the smaller tile of the two forms
 // With fewer than kPackedRoots roots (so every label + 1 < 2^17) a pair is
 // one word: (label + 1) << 15 | position within the bucket (B <= 2^15); else
 // two words, input position and label.  The form is read from num_roots on
-// the device; both forms are launched and the other returns at once.
+// the device, in the kernels themselves.
 constexpr int32_t kPackedRoots = (1 << 17) - 1;
 template <bool kPacked>
 struct LabelForm {
-  static constexpr int kThreads = kPacked ? 1024 : 512;
-  static constexpr int kTile = kThreads * kLabelItems;
+  static constexpr int kItems = kPacked ? 16 : 8;
+  static constexpr int kTile = kLabelThreads * kItems;
   using Slot = std::conditional_t<kPacked, uint32_t, uint64_t>;  // 64 KB of LDS either way
 };
-static_assert(LabelForm<false>::kThreads >= kMaxBuckets, "a thread per bucket");
+static_assert(kLabelThreads >= kMaxBuckets, "a thread per bucket");
 constexpr uint64_t kLabelTagAgg = 17ull << 48, kLabelTagPre = 18ull << 48;  // after the passes' tags
 
 // Pass A: every sorted point's label (a core point: its cell's cluster id;
@@ -2236,22 +2236,18 @@ constexpr uint64_t kLabelTagAgg = 17ull << 48, kLabelTagPre = 18ull << 48;  // a
 // thread), so the runs leave as contiguous writes (16 lanes per run); the
 // order inside a bucket is free.
 template <bool kPacked>
-__global__ __launch_bounds__(LabelForm<kPacked>::kThreads) void k_label(
+__device__ __forceinline__ void label_tile(
     Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
     const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
     const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
     const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
     uint32_t* __restrict__ spare_xa, uint32_t* __restrict__ spare_xb,
     uint32_t* __restrict__ spare_ya, uint32_t* __restrict__ spare_yb,
-    uint64_t* __restrict__ status, int64_t n64, uint32_t B) {
+    uint64_t* __restrict__ status, int64_t n64, uint32_t B, uint32_t* s_cnt, uint32_t* s_len,
+    uint32_t* s_base, typename LabelForm<kPacked>::Slot* s_buf, uint32_t* s_scan) {
   using F = LabelForm<kPacked>;
-  constexpr int T = F::kThreads;
-  __shared__ uint32_t s_cnt[kMaxBuckets];  // counts, then the runs' local starts
-  __shared__ uint32_t s_len[kMaxBuckets];
-  __shared__ uint32_t s_base[kMaxBuckets];  // the runs' global starts
-  __shared__ typename F::Slot s_buf[F::kTile];
-  __shared__ uint32_t s_scan[T / 64];
-  if ((prm->num_roots < kPackedRoots) != kPacked) return;  // the other form's launch
+  constexpr int T = kLabelThreads;
+  constexpr int kLabelItems = F::kItems;
   const uint32_t n = static_cast<uint32_t>(n64);
   const bool in_a = (prm->npasses & 1) != 0;
   const uint32_t* __restrict__ idx = in_a ? idx_a : idx_b;
@@ -2333,6 +2329,32 @@ __global__ __launch_bounds__(LabelForm<kPacked>::kThreads) void k_label(
   }
 }
 
+__global__ __launch_bounds__(kLabelThreads) void k_label(
+    Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
+    const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
+    const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
+    const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
+    uint32_t* __restrict__ spare_xa, uint32_t* __restrict__ spare_xb,
+    uint32_t* __restrict__ spare_ya, uint32_t* __restrict__ spare_yb,
+    uint64_t* __restrict__ status, int64_t n64, uint32_t B) {
+  __shared__ uint32_t s_cnt[kMaxBuckets];  // counts, then the runs' local starts
+  __shared__ uint32_t s_len[kMaxBuckets];
+  __shared__ uint32_t s_base[kMaxBuckets];  // the runs' global starts
+  __shared__ uint64_t s_raw[kLabelTile];    // the tile's pairs, 64 KB in either form
+  __shared__ uint32_t s_scan[kLabelThreads / 64];
+  if (prm->num_roots < kPackedRoots) {
+    // the grid counts the two-word form's tiles; the packed form has half
+    if (static_cast<int64_t>(blockIdx.x) * LabelForm<true>::kTile >= n64) return;
+    label_tile<true>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, spare_ia, spare_ib,
+                     spare_xa, spare_xb, spare_ya, spare_yb, status, n64, B, s_cnt, s_len,
+                     s_base, reinterpret_cast<uint32_t*>(s_raw), s_scan);
+  } else {
+    label_tile<false>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, spare_ia, spare_ib,
+                      spare_xa, spare_xb, spare_ya, spare_yb, status, n64, B, s_cnt, s_len,
+                      s_base, s_raw, s_scan);
+  }
+}
+
 // Inputs of more than kMaxBuckets * kUnpermMaxWidth points: labels written
 // straight to their input positions (one 4-byte write per label).
 __global__ __launch_bounds__(256) void k_label_direct(
@@ -2353,14 +2375,14 @@ __global__ __launch_bounds__(256) void k_label_direct(
 
 // Pass B: one workgroup per bucket places the bucket's labels in LDS by input
 // position and writes them out in order (B <= kWidth).
-template <int kWidth, int kThreads, bool kPacked>
+template <int kWidth, int kThreads>
 __global__ __launch_bounds__(kThreads) void k_unpermute(
     const Params* __restrict__ prm, const uint32_t* __restrict__ spare_xa,
     const uint32_t* __restrict__ spare_xb, const uint32_t* __restrict__ spare_ya,
     const uint32_t* __restrict__ spare_yb, int32_t* __restrict__ labels, int64_t n64,
     uint32_t B) {
   __shared__ int32_t s_lab[kWidth];
-  if ((prm->num_roots < kPackedRoots) != kPacked) return;  // the other form's launch
+  const bool packed = prm->num_roots < kPackedRoots;
   const uint32_t n = static_cast<uint32_t>(n64);
   const bool in_a = (prm->npasses & 1) != 0;
   const uint32_t* __restrict__ pair_i = in_a ? spare_xb : spare_xa;
@@ -2369,7 +2391,7 @@ __global__ __launch_bounds__(kThreads) void k_unpermute(
   const uint32_t b0 = static_cast<uint32_t>(blockIdx.x) * B;
   const uint32_t b1 = min(n, b0 + min(B, static_cast<uint32_t>(kWidth)));
   for (uint32_t j = b0 + t; j < b1; j += kThreads) {
-    if constexpr (kPacked) {
+    if (packed) {
       const uint32_t v = pair_i[j];
       const uint32_t d = b0 + (v & 0x7FFFu);
       if (d < b1) s_lab[d - b0] = static_cast<int32_t>(v >> 15) - 1;
@@ -2629,34 +2651,25 @@ void launch_unpermute(uint32_t B, int64_t n, hipStream_t stream, const Params* p
                       float* xs_a, float* xs_b, float* ys_a, float* ys_b, int32_t* labels) {
   const int nb = static_cast<int>((n + B - 1) / B);
   auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
-  if (B <= static_cast<uint32_t>(kUnpermSmallWidth)) {
-    k_unpermute<kUnpermSmallWidth, 512, true><<<nb, 512, 0, stream>>>(
-        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
-    k_unpermute<kUnpermSmallWidth, 512, false><<<nb, 512, 0, stream>>>(
-        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
-  } else {
-    k_unpermute<kUnpermMaxWidth, 1024, true><<<nb, 1024, 0, stream>>>(
-        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
-    k_unpermute<kUnpermMaxWidth, 1024, false><<<nb, 1024, 0, stream>>>(
-        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), labels, n, B);
-  }
+  if (B <= static_cast<uint32_t>(kUnpermSmallWidth))
+    k_unpermute<kUnpermSmallWidth, 512><<<nb, 512, 0, stream>>>(prm, u(xs_a), u(xs_b), u(ys_a),
+                                                                 u(ys_b), labels, n, B);
+  else
+    k_unpermute<kUnpermMaxWidth, 1024><<<nb, 1024, 0, stream>>>(prm, u(xs_a), u(xs_b), u(ys_a),
+                                                                 u(ys_b), labels, n, B);
 }
 
 int grid_for(int64_t items, int per_block, int cap);
 
-// Both forms of the label pass (the one that does not match num_roots
-// returns at once).
+// The label pass (its form chosen on the device).
 void launch_label(uint32_t B, int64_t n, hipStream_t stream, Params* prm, const uint32_t* idx_a,
                   const uint32_t* idx_b, const int32_t* point_cell, const uint8_t* is_core,
                   const int32_t* cell_cluster, float* xs_a, float* xs_b, float* ys_a,
                   float* ys_b, uint64_t* status) {
   auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
-  k_label<true><<<grid_for(n, LabelForm<true>::kTile, INT_MAX), LabelForm<true>::kThreads, 0,
-                  stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
-                            u(xs_a), u(xs_b), u(ys_a), u(ys_b), status, n, B);
-  k_label<false><<<grid_for(n, LabelForm<false>::kTile, INT_MAX), LabelForm<false>::kThreads, 0,
-                   stream>>>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b,
-                             u(xs_a), u(xs_b), u(ys_a), u(ys_b), status, n, B);
+  k_label<<<grid_for(n, kLabelTile, INT_MAX), kLabelThreads, 0, stream>>>(
+      prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, u(xs_a), u(xs_b),
+      u(ys_a), u(ys_b), status, n, B);
 }
 
 int grid_for(int64_t items, int per_block, int cap) {
