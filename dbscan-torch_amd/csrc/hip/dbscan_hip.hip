@@ -1462,7 +1462,7 @@ __global__ __launch_bounds__(256) void k_bbox(const Params* __restrict__ prm,
 
 // Cells up to this many points are scanned by a single lane; larger cells are
 // handed to the whole wave.
-constexpr int32_t kLaneScan = 256;
+constexpr int32_t kLaneScan = 64;
 
 // Counts the points of [cs, ce) within eps of (x, y) for one lane, four
 // candidates per step, until `total` reaches ms.
