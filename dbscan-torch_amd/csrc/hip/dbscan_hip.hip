@@ -76,6 +76,7 @@ constexpr int kMaxReduceBlocks = 512;
 // is bounded by kSpinLimit and reports kErrSortSpin / kErrScanSpin instead of
 // hanging.
 constexpr uint32_t kSpinLimit = 1u << 26;         // bounded waits; see ErrFlag
+constexpr int kTailPass = 4;  // radix passes from here on: one walking launch (k_sort_tail)
 constexpr uint64_t kCountMask = (1ull << 48) - 1;
 
 // Thread t < kRadixMax of a sort workgroup owns digits [t*D, (t+1)*D).
@@ -692,7 +693,7 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 // look-back stops at the cloud's first tile, and the cloud's points stay in
 // its own range [begin, end), so every cloud is sorted on its own.
 template <bool kFirst, bool kSeg, int kItems, bool kWalk>
-__global__ __launch_bounds__(kSortThreads) void k_sort_pass(
+__device__ __forceinline__ void sort_pass_tiles(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
@@ -885,6 +886,66 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(
     }
     __syncthreads();
     if constexpr (!kWalk) break;
+  }
+}
+
+// One radix pass, one tile per workgroup.
+template <bool kFirst, bool kSeg, int kItems>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(
+    const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
+    const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
+    uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
+    int pass, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
+    const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist,
+    uint32_t num_tiles) {
+  sort_pass_tiles<kFirst, kSeg, kItems, false>(X, in_x, in_y, in_i, out_x, out_y, out_i, prm,
+                                               status, pass, n64, inv_r, tiles, clouds,
+                                               cloud_hist, num_tiles);
+}
+
+// All workgroups of the grid meet here; writes before it are visible to
+// every workgroup after it (agent-scope release and acquire around an atomic
+// count).  Needs every workgroup resident, as the walking passes do; bounded
+// by kSpinLimit.
+__device__ void grid_barrier(uint32_t* counter, uint32_t target, int32_t* error) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(counter, 1u);
+    uint32_t spins = 0;
+    while (ld_acquire(counter) < target) {
+      if (++spins > kSpinLimit) {
+        atomicOr(error, static_cast<int32_t>(kErrSortSpin));
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+// The radix passes from kTailPass on, all in one launch of one workgroup per
+// CU (they are needed only for keys wider than 32 bits; for the usual key the
+// launch returns at once): each pass walks the tiles, and a grid barrier
+// separates the passes.  Pass p reads buffer set a if p is odd, else b.
+template <bool kSeg, int kItems>
+__global__ __launch_bounds__(kSortThreads) void k_sort_tail(
+    float* __restrict__ xs_a, float* __restrict__ ys_a, uint32_t* __restrict__ idx_a,
+    float* __restrict__ xs_b, float* __restrict__ ys_b, uint32_t* __restrict__ idx_b,
+    Params* __restrict__ prm, uint64_t* __restrict__ status, int64_t n64, float inv_r,
+    const SegTile* __restrict__ tiles, const CloudInfo* __restrict__ clouds,
+    const uint32_t* __restrict__ cloud_hist, uint32_t num_tiles) {
+  const int np = prm->npasses;
+  for (int pass = kTailPass; pass < np; ++pass) {
+    const bool from_a = (pass & 1) != 0;
+    sort_pass_tiles<false, kSeg, kItems, true>(
+        nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
+        from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, status, pass,
+        n64, inv_r, tiles, clouds, cloud_hist, num_tiles);
+    if (pass + 1 < np)
+      grid_barrier(&prm->tile_counter[0], gridDim.x * static_cast<uint32_t>(pass - kTailPass + 1),
+                   &prm->error);
   }
 }
 
@@ -2737,13 +2798,13 @@ bool config_ok(const dbscan_hip_config& c) {
 }
 
 // Passes from kTailPass on are needed only for keys wider than 32 bits
-// (4 x 8-bit digits); they are launched with one workgroup per CU that walks
-// the tiles (kWalk), so that for the usual narrow key each is a small launch
-// rather than one workgroup per tile.  One per CU is always co-resident, which
-// the walk needs: a workgroup waits only on tiles of lower index, held by
-// workgroups that are resident or done.  (The walk is a separate
-// instantiation: its loop would cost the one-tile kernel registers.)
-constexpr int kTailPass = 4;
+// (4 x 8-bit digits); they run in one launch (k_sort_tail) of one workgroup
+// per CU that walks the tiles, so that for the usual narrow key they cost a
+// single launch that returns at once.  One workgroup per CU is always
+// co-resident, which the walk and the barrier between passes need: a
+// workgroup waits only on tiles of lower index, held by workgroups that are
+// resident or done.  (The walk is a separate instantiation: its loop would
+// cost the one-tile kernel registers.)
 
 int tail_grid(int tiles) {
   int dev = 0, cus = 0;
@@ -2756,24 +2817,27 @@ int tail_grid(int tiles) {
 
 // Radix pass `pass` with `items` points per thread over `tiles` tiles.
 template <bool kFirst, bool kSeg, class... A>
-void launch_sort(int items, int pass, int tiles, hipStream_t st, A... a) {
+void launch_sort(int items, int tiles, hipStream_t st, A... a) {
   const uint32_t nt = static_cast<uint32_t>(tiles);
-  if (!kFirst && pass >= kTailPass) {
-    const int g = tail_grid(tiles);
-    if (items == 8)
-      k_sort_pass<kFirst, kSeg, 8, true><<<g, kSortThreads, 0, st>>>(a..., nt);
-    else if (items == 16)
-      k_sort_pass<kFirst, kSeg, 16, true><<<g, kSortThreads, 0, st>>>(a..., nt);
-    else
-      k_sort_pass<kFirst, kSeg, 12, true><<<g, kSortThreads, 0, st>>>(a..., nt);
-    return;
-  }
   if (items == 8)
-    k_sort_pass<kFirst, kSeg, 8, false><<<tiles, kSortThreads, 0, st>>>(a..., nt);
+    k_sort_pass<kFirst, kSeg, 8><<<tiles, kSortThreads, 0, st>>>(a..., nt);
   else if (items == 16)
-    k_sort_pass<kFirst, kSeg, 16, false><<<tiles, kSortThreads, 0, st>>>(a..., nt);
+    k_sort_pass<kFirst, kSeg, 16><<<tiles, kSortThreads, 0, st>>>(a..., nt);
   else
-    k_sort_pass<kFirst, kSeg, 12, false><<<tiles, kSortThreads, 0, st>>>(a..., nt);
+    k_sort_pass<kFirst, kSeg, 12><<<tiles, kSortThreads, 0, st>>>(a..., nt);
+}
+
+// Passes kTailPass.. in one launch (k_sort_tail).
+template <bool kSeg, class... A>
+void launch_sort_tail(int items, int tiles, hipStream_t st, A... a) {
+  const uint32_t nt = static_cast<uint32_t>(tiles);
+  const int g = tail_grid(tiles);
+  if (items == 8)
+    k_sort_tail<kSeg, 8><<<g, kSortThreads, 0, st>>>(a..., nt);
+  else if (items == 16)
+    k_sort_tail<kSeg, 16><<<g, kSortThreads, 0, st>>>(a..., nt);
+  else
+    k_sort_tail<kSeg, 12><<<g, kSortThreads, 0, st>>>(a..., nt);
 }
 
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
@@ -2881,16 +2945,20 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   // Stage 2: radix passes (sort + gather in one).
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  launch_sort<true, false>(cfg.sort_items, 0, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
+  launch_sort<true, false>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
                            ys_a, idx_a, prm, sort_status, 0, n, inv_r, nullptr, nullptr,
                            nullptr);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    launch_sort<false, false>(
-        cfg.sort_items, p, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
-        from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
-        from_a ? idx_b : idx_a, prm, sort_status, p, n, inv_r, nullptr, nullptr, nullptr);
+    if (p < kTailPass)
+      launch_sort<false, false>(
+          cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
+          from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
+          from_a ? idx_b : idx_a, prm, sort_status, p, n, inv_r, nullptr, nullptr, nullptr);
+    else if (p == kTailPass)
+      launch_sort_tail<false>(cfg.sort_items, g_sort, stream, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b,
+                              prm, sort_status, n, inv_r, nullptr, nullptr, nullptr);
   }
 
   // Stage 3: cell runs.
@@ -3106,16 +3174,20 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   k_hist_seg<<<g_hist, 256, 0, stream>>>(X2, seg_sort, g_sort, hist_per_block, clouds, prm,
                                          cloud_hist, col_range);
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
-  launch_sort<true, true>(cfg.sort_items, 0, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
+  launch_sort<true, true>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
                           ys_a, idx_a, prm, sort_status, 0, n, 0.f, seg_sort, clouds,
                           cloud_hist);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    launch_sort<false, true>(
-        cfg.sort_items, p, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
-        from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
-        from_a ? idx_b : idx_a, prm, sort_status, p, n, 0.f, seg_sort, clouds, cloud_hist);
+    if (p < kTailPass)
+      launch_sort<false, true>(
+          cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
+          from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
+          from_a ? idx_b : idx_a, prm, sort_status, p, n, 0.f, seg_sort, clouds, cloud_hist);
+    else if (p == kTailPass)
+      launch_sort_tail<true>(cfg.sort_items, g_sort, stream, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b,
+                             prm, sort_status, n, 0.f, seg_sort, clouds, cloud_hist);
   }
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
   k_cells<true><<<g_scan_pts, kScanThreads, 0, stream>>>(

@@ -149,6 +149,27 @@ def test_back_to_back_calls_reuse_the_workspace(dt):
         assert torch.equal(dt.dbscan2d(Xd, 0.5, 1), ref), it
 
 
+def test_wide_keys_take_the_tail_passes(dt):
+    """Cell keys wider than 32 bits (x and y spread over 2e6 with eps 0.01:
+    about 29 bits each) need more than four radix passes, run by the walking
+    tail launch with a grid barrier between passes; single and batched calls
+    against the oracle."""
+    rng = np.random.default_rng(41)
+    far = rng.uniform(-1e6, 1e6, (40_000, 2))
+    blobs = np.concatenate([rng.normal(c, 0.004, (500, 2)) for c in rng.uniform(-1e6, 1e6, (20, 2))])
+    X = np.concatenate([far, blobs]).astype(np.float32)
+    X = X[rng.permutation(len(X))]
+    dbg = dt._C.dbscan2d_hip_debug(torch.from_numpy(X).to(DEV), 0.01, 5)
+    assert int(dbg[1][5].item()) > 4  # radix passes
+    ref = oracle.dbscan2d(X, 0.01, 5)
+    assert np.array_equal(gpu_labels(dt, X, 0.01, 5), ref)
+    small = X[:3000].copy()
+    out = dt.dbscan2d_batched([torch.from_numpy(small).to(DEV), torch.from_numpy(X).to(DEV)],
+                              [0.5, 0.01], [3, 5])
+    assert np.array_equal(out[1].cpu().numpy(), ref)
+    assert np.array_equal(out[0].cpu().numpy(), oracle.dbscan2d(small, 0.5, 3))
+
+
 def test_permutation_equivariance_lattice1m(dt):
     X, eps, ms = W.lattice1m()
     base = gpu_labels(dt, X, eps, ms)
