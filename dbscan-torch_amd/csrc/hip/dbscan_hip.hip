@@ -67,6 +67,10 @@ constexpr int kSortTileMin = kSortThreads * 8;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
+// The cell pass's tile (a batch's tiles: up to kCellTile points of one cloud).
+constexpr int kCellItems = 8;
+constexpr int kCellTile = kScanThreads * kCellItems;
+static_assert(kCellTile >= kScanTile, "look-back words are sized for kScanTile tiles");
 constexpr int kMaxReduceBlocks = 512;
 // kTileOrder: the look-back kernels (k_sort_pass, k_cells, k_rank) use the
 // workgroup id as the tile index.  Workgroups of a 1-D grid are dispatched in
@@ -951,7 +955,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
 
 // Cell runs over the sorted points (dbscan_grid.hpp:206-227): flags where the
 // key changes, block scan plus look-back, then point->cell, cell starts and
-// cell keys.  Blocked layout: thread t owns kScanItems consecutive points,
+// cell keys.  Blocked layout: thread t owns kCellItems consecutive points,
 // read and written with 16-byte vector accesses where aligned.
 // kSeg (batches): tile t is tiles[t], a run of one
 // cloud's points; a cloud's first point always opens a cell, keys are the
@@ -966,7 +970,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     uint32_t* __restrict__ cell_flags, uint4* __restrict__ cell_box, int2* __restrict__ col_range,
     int64_t max_cols, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
     CloudInfo* __restrict__ clouds, int32_t* __restrict__ cell_cloud) {
-  static_assert(kScanItems == 8, "two float4 per thread");
+  static_assert(kCellItems % 4 == 0, "whole float4s per thread");
   __shared__ uint64_t s_last[kScanThreads];
   __shared__ uint32_t s_scan[kScanThreads / 64];
   __shared__ uint64_t s_prefix;
@@ -995,41 +999,40 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     col_base = ci.col_base;
   } else {
     m = load_keymap(prm);
-    base = tile * kScanTile;
-    end = min(base + kScanTile, n);
+    base = tile * kCellTile;
+    end = min(base + kCellTile, n);
     cbeg = 0;
     cend = n;
     colidx = has_col_index(m, max_cols);
   }
-  const uint32_t e0 = base + static_cast<uint32_t>(t) * kScanItems;
-  const bool whole = e0 + kScanItems <= end;
+  const uint32_t e0 = base + static_cast<uint32_t>(t) * kCellItems;
+  const bool whole = e0 + kCellItems <= end;
 
-  float px[kScanItems], py[kScanItems];
+  float px[kCellItems], py[kCellItems];
   if (whole && (e0 & 3u) == 0) {
-    const float4 x0 = *reinterpret_cast<const float4*>(xs + e0);
-    const float4 x1 = *reinterpret_cast<const float4*>(xs + e0 + 4);
-    const float4 y0 = *reinterpret_cast<const float4*>(ys + e0);
-    const float4 y1 = *reinterpret_cast<const float4*>(ys + e0 + 4);
-    px[0] = x0.x; px[1] = x0.y; px[2] = x0.z; px[3] = x0.w;
-    px[4] = x1.x; px[5] = x1.y; px[6] = x1.z; px[7] = x1.w;
-    py[0] = y0.x; py[1] = y0.y; py[2] = y0.z; py[3] = y0.w;
-    py[4] = y1.x; py[5] = y1.y; py[6] = y1.z; py[7] = y1.w;
+#pragma unroll
+    for (int v = 0; v < kCellItems / 4; ++v) {
+      const float4 xv = *reinterpret_cast<const float4*>(xs + e0 + 4 * v);
+      const float4 yv = *reinterpret_cast<const float4*>(ys + e0 + 4 * v);
+      px[4 * v] = xv.x; px[4 * v + 1] = xv.y; px[4 * v + 2] = xv.z; px[4 * v + 3] = xv.w;
+      py[4 * v] = yv.x; py[4 * v + 1] = yv.y; py[4 * v + 2] = yv.z; py[4 * v + 3] = yv.w;
+    }
   } else {
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
+    for (int j = 0; j < kCellItems; ++j) {
       const uint32_t ec = e0 + j < end ? e0 + j : end - 1;  // clamped: defined, never used
       px[j] = xs[ec];
       py[j] = ys[ec];
     }
   }
-  uint64_t key[kScanItems];
+  uint64_t key[kCellItems];
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
+  for (int j = 0; j < kCellItems; ++j) {
     uint32_t xr, yr;
     rel_cell(px[j], py[j], inv_r, m, xr, yr);
     key[j] = cell_key64(xr, yr);
   }
-  s_last[t] = key[kScanItems - 1];
+  s_last[t] = key[kCellItems - 1];
   __syncthreads();
   uint64_t prev = 0;
   bool have_prev = false;  // the previous point exists and is of the same cloud
@@ -1043,9 +1046,9 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     have_prev = true;
   }
   uint32_t flags = 0, newcol = 0, hadprev = 0, cnt = 0;
-  uint32_t prev_x[kScanItems];
+  uint32_t prev_x[kCellItems];
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
+  for (int j = 0; j < kCellItems; ++j) {
     const bool f = e0 + j < end && (!have_prev || key[j] != prev);
     flags |= static_cast<uint32_t>(f) << j;
     newcol |= static_cast<uint32_t>(f && (!have_prev || (key[j] >> 32) != (prev >> 32))) << j;
@@ -1063,9 +1066,9 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
   }
   __syncthreads();
   int32_t c = static_cast<int32_t>(s_prefix + texcl) - 1;
-  int32_t pc[kScanItems];
+  int32_t pc[kCellItems];
 #pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
+  for (int j = 0; j < kCellItems; ++j) {
     if ((flags >> j) & 1u) {
       ++c;
       if (static_cast<uint32_t>(c) < n) {
@@ -1088,14 +1091,16 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     pc[j] = c;
   }
   if (whole && (e0 & 3u) == 0) {
-    *reinterpret_cast<int4*>(point_cell + e0) = make_int4(pc[0], pc[1], pc[2], pc[3]);
-    *reinterpret_cast<int4*>(point_cell + e0 + 4) = make_int4(pc[4], pc[5], pc[6], pc[7]);
+#pragma unroll
+    for (int v = 0; v < kCellItems / 4; ++v)
+      *reinterpret_cast<int4*>(point_cell + e0 + 4 * v) =
+          make_int4(pc[4 * v], pc[4 * v + 1], pc[4 * v + 2], pc[4 * v + 3]);
   } else {
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j)
+    for (int j = 0; j < kCellItems; ++j)
       if (e0 + j < end) point_cell[e0 + j] = pc[j];
   }
-  if (e0 <= cend - 1 && cend - 1 < e0 + kScanItems && cend - 1 < end) {
+  if (e0 <= cend - 1 && cend - 1 < e0 + kCellItems && cend - 1 < end) {
     // this thread holds the cloud's last point
     const int j = static_cast<int>((cend - 1) - e0);
     const int32_t last = pc[j];
@@ -2577,6 +2582,25 @@ __global__ __launch_bounds__(256) void k_border(
       s = cell_start[c];
       e = cell_start[c + 1];
       cp = pp.at(c);
+      // no core cell in the 25-cell window: every point of the cell is noise
+      // (two rounds of loads for the cell rather than a search per point)
+      const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+      int32_t j[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) j[k] = row[k];
+      uint32_t fl[25];
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+          fl[5 * k + q] = j[2 * k] + q < j[2 * k + 1] ? cell_flags[j[2 * k] + q] : 0u;
+      uint32_t any = 0;
+#pragma unroll
+      for (int k = 0; k < 25; ++k) any |= fl[k];
+      if (!(any & kCellHasCore)) {
+        for (int32_t p = s; p < e; ++p) lab_sorted[p] = kNoise;
+        has = false;
+      }
     }
     const float eps_sq = cp.eps_sq, best_init = cp.best_init;
     // The points of these cells are shared out one per lane (work item q of
@@ -2929,6 +2953,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const int g_hist = grid_for(n, 16 * kHistThreads, kHistBlocks);
   const int g_sort = grid_for(n, kSortThreads * cfg.sort_items, INT_MAX);
   const int g_scan = grid_for(n, kScanTile, INT_MAX);
+  const int g_celltiles = grid_for(n, kCellTile, INT_MAX);
   const int g_cells = grid_for(n, 256, 16384);
   const int g_cellwaves = grid_for(n, 256, cfg.cell_blocks);  // one lane per cell, grid-stride
   const int g_union = grid_for(n * 7, 256, cfg.pair_blocks);
@@ -2963,7 +2988,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   // Stage 3: cell runs.
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
-  k_cells<false><<<g_scan, kScanThreads, 0, stream>>>(
+  k_cells<false><<<g_celltiles, kScanThreads, 0, stream>>>(
       xs_a, ys_a, xs_b, ys_b, prm, at<uint64_t>(ws, L.cell_status), point_cell, cell_start,
       cell_key, cell_flags, cell_box, at<int2>(ws, L.col_range), L.max_cols, n, inv_r, nullptr,
       nullptr, nullptr);
@@ -3086,7 +3111,7 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
     ci.col_base = -1;
     const int64_t ni = offsets[i + 1] - offsets[i];
     sort_tiles += (ni + sort_tile - 1) / sort_tile;
-    scan_tiles += (ni + kScanTile - 1) / kScanTile;
+    scan_tiles += (ni + kCellTile - 1) / kCellTile;
   }
   if (total == 0) return DBSCAN_HIP_OK;
   if (!X || !labels || !workspace) return DBSCAN_HIP_ERR_NULL;
@@ -3108,8 +3133,8 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
       const int32_t first = a;
       for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += sort_tile)
         st_sort[a++] = SegTile{i, p0, std::min(sort_tile, info[i].end - p0), first};
-      for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kScanTile)
-        st_scan[b++] = SegTile{i, p0, std::min(kScanTile, info[i].end - p0), 0};
+      for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kCellTile)
+        st_scan[b++] = SegTile{i, p0, std::min(kCellTile, info[i].end - p0), 0};
     }
   }
 
