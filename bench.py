@@ -67,8 +67,7 @@ KERNEL_BYTES = {
     "k_cells": (12, 12), "k_bbox": (13, 16), "k_core_sparse": (0, 12),
     "k_label": (13, 0), "k_unpermute": (8, 0), "k_border": (0, 12),
     "k_ngh": (0, 48), "k_union[3x3]": (9, 16), "k_union_local": (0, 52),
-    "k_flatten[3x3]": (0, 12),
-    "k_union[rest]": (0, 24), "k_flatten": (0, 12), "k_rank": (0, 8),
+    "k_union[rest]": (0, 24), "k_flatten": (0, 12), "k_rank": (0, 12),
     "k_cluster": (0, 16),
 }
 

@@ -2194,22 +2194,26 @@ __global__ __launch_bounds__(256) void k_flatten(const Params* __restrict__ prm,
                                                  const int32_t* __restrict__ cell_ref,
                                                  const uint32_t* __restrict__ cell_flags,
                                                  int32_t* __restrict__ uf,
-                                                 uint32_t* __restrict__ root_flag) {
+                                                 uint32_t* __restrict__ core_ref) {
   const int32_t C = prm->num_cells;
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
        c += static_cast<int64_t>(gridDim.x) * 256) {
     const int32_t r = cell_ref[c];
     const int32_t root = uf_find_ro(uf, r);
-    root_flag[r] = (root == r && (cell_flags[c] & 1u)) ? 1u : 0u;
+    core_ref[r] = cell_flags[c] & 1u;  // the cell's core bit, by reference index
     st_release(uf + r, root);
   }
 }
 
 // Exclusive scan of the root flags over reference indices, in place: the
-// rank of a root is its cluster id (dbscan_grid.hpp:392-398).
+// rank of a root is its cluster id (dbscan_grid.hpp:392-398).  On entry
+// flag_rank holds each reference index's core bit (k_flatten); a root is a
+// core cell r with uf[r] == r (the unions after the flatten leave uf
+// unflattened; its roots are still its fixed points).
 __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
                                                        uint64_t* __restrict__ status,
-                                                       uint32_t* __restrict__ flag_rank) {
+                                                       uint32_t* __restrict__ flag_rank,
+                                                       const int32_t* __restrict__ uf) {
   __shared__ uint32_t s_scan[kScanThreads / 64];
   __shared__ uint64_t s_prefix;
   const int64_t C = prm->num_cells;
@@ -2224,7 +2228,9 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
   uint32_t cnt = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    f[j] = (e0 + j < C) ? flag_rank[e0 + j] : 0u;
+    f[j] = (e0 + j < C && ld_acquire(uf + e0 + j) == static_cast<int32_t>(e0 + j))
+               ? flag_rank[e0 + j]
+               : 0u;
     cnt += f[j];
   }
   uint32_t ttot;
@@ -2249,7 +2255,7 @@ template <bool kSeg>
 __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
                                                  const int32_t* __restrict__ cell_ref,
                                                  const uint32_t* __restrict__ cell_flags,
-                                                 const int32_t* __restrict__ uf,
+                                                 int32_t* __restrict__ uf,
                                                  const uint32_t* __restrict__ rank,
                                                  int32_t* __restrict__ cell_cluster,
                                                  const int32_t* __restrict__ cell_cloud,
@@ -2258,8 +2264,13 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
        c += static_cast<int64_t>(gridDim.x) * 256) {
     int32_t id = kNoise;
+    // the root by a read-only walk, written back (uf ends flattened; every
+    // value written is a root, so concurrent walks stay on valid ancestors)
+    const int32_t r = cell_ref[c];
+    const int32_t root = uf_find_ro(uf, r);
+    if (root != r) st_release(uf + r, root);
     if (cell_flags[c] & 1u) {
-      id = static_cast<int32_t>(rank[uf[cell_ref[c]]]);
+      id = static_cast<int32_t>(rank[root]);
       if constexpr (kSeg) id -= static_cast<int32_t>(rank[clouds[cell_cloud[c]].cell_begin]);
     }
     cell_cluster[c] = id;
@@ -2783,7 +2794,6 @@ enum KernelSlot : int {
   kKUnionLocal,
   kKFlatten0,
   kKUnion1,
-  kKFlatten,
   kKRank,
   kKCluster,
   kKBorder,
@@ -2796,7 +2806,7 @@ static_assert(kNumKernels == DBSCAN_HIP_NUM_KERNELS, "kernel count in the header
 const char* const kKernelNames[kNumKernels] = {
     "k_bounds", "k_hist", "k_sort_pass[0]", "k_sort_pass[1]", "k_sort_pass[2]",
     "k_sort_pass[3]", "k_sort_pass[4]", "k_sort_pass[5]", "k_sort_pass[6]", "k_sort_pass[7]",
-    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_union_local", "k_flatten[3x3]", "k_union[rest]", "k_flatten", "k_rank", "k_cluster", "k_border", "k_label", "k_unpermute"};
+    "k_cells", "k_bbox", "k_ngh", "k_core_sparse", "k_union[3x3]", "k_union_local", "k_flatten", "k_union[rest]", "k_rank", "k_cluster", "k_border", "k_label", "k_unpermute"};
 
 // Kernel slots [begin, end) of each reference stage.  Border assignment runs
 // before the core labels here: it writes sorted-order labels that the label
@@ -3022,12 +3032,10 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
                                           cell_flags, cell_box, cell_key, is_core, uf, pp, xs_a,
                                           xs_b);
-  if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
-  k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
 
   // Stage 7: cluster ids.
   if ((e = before(kKRank)) != hipSuccess) return fail(e);
-  k_rank<<<g_scan, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank);
+  k_rank<<<g_scan, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank, uf);
   if ((e = before(kKCluster)) != hipSuccess) return fail(e);
   k_cluster<false><<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank,
                                                 cell_cluster, nullptr, nullptr);
@@ -3242,10 +3250,9 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   k_union<1><<<g_union, 256, 0, stream>>>(prm, xs_a, ys_a, xs_b, ys_b, cell_start, ngh, cell_ref,
                                           cell_flags, cell_box, cell_key, is_core, uf, pp, xs_a,
                                           xs_b);
-  if ((e = before(kKFlatten)) != hipSuccess) return fail(e);
-  k_flatten<<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank);
   if ((e = before(kKRank)) != hipSuccess) return fail(e);
-  k_rank<<<g_scan_cells, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank);
+  k_rank<<<g_scan_cells, kScanThreads, 0, stream>>>(prm, at<uint64_t>(ws, L.rank_status), rank,
+                                                    uf);
   if ((e = before(kKCluster)) != hipSuccess) return fail(e);
   k_cluster<true><<<g_cells, 256, 0, stream>>>(prm, cell_ref, cell_flags, uf, rank, cell_cluster,
                                                cell_cloud, clouds);

@@ -107,7 +107,7 @@ const char* dbscan2d_hip_stage_name(int i);
 
 /* Number of kernel launch slots of one call (sort passes beyond the key width
  * are launched and return at once), and their names. */
-#define DBSCAN_HIP_NUM_KERNELS 24
+#define DBSCAN_HIP_NUM_KERNELS 23
 const char* dbscan2d_hip_kernel_name(int i);
 
 /*
