@@ -227,8 +227,10 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 192) -> dict:
     server = Server(slots=3, slot_points=32 << 20, slot_clouds=4096)
 
     def make(rate):
-        arr, _, eps, ms = W.serve_stream(rate, count)
-        reqs = [Request(i, clouds[i % pool], float(eps[i]), int(ms[i])) for i in range(count)]
+        # about 4 s of stream per rate (at least 200 requests, at most `count`)
+        k = min(count, max(200, int(rate * 4)))
+        arr, _, eps, ms = W.serve_stream(rate, k)
+        reqs = [Request(i, clouds[i % pool], float(eps[i]), int(ms[i])) for i in range(k)]
         return arr, reqs
 
     # warm-up: one short stream
@@ -236,7 +238,7 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 192) -> dict:
     run_open_loop(server, reqs[:100], arr[:100])
     out = {"workload": "serve: open-loop Poisson, N mixture 70% log-U[1k,50k], 25% "
                        "log-U[50k,1M], 5% log-U[1M,10M], eps U[0.2,1.0], min_samples {5,10,20}",
-           "requests_per_rate": count, "pool_clouds": pool,
+           "max_requests_per_rate": count, "pool_clouds": pool,
            "arena_gb": round(server.arena_bytes / 1e9, 2), "slots": len(server.slots),
            "rates": {}}
     for rate in rates:
@@ -285,10 +287,10 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-only", action="store_true",
                     help="run only the serving leg (prints its JSON)")
-    ap.add_argument("--serve-rates", default="500,1000,2000,5000",
+    ap.add_argument("--serve-rates", default="100,500,1000,2000,5000",
                     help="comma-separated request rates (req/s) of the serving leg")
     ap.add_argument("--serve-requests", type=int, default=2000,
-                    help="requests per rate in the serving leg")
+                    help="most requests per rate in the serving leg (about 4 s of stream)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
