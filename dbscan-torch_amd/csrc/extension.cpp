@@ -422,6 +422,35 @@ dbscan2d_batched_benchmark(torch::Tensor X, std::vector<int64_t> offsets, std::v
   return {labels, r.first, kernel_names(), r.second};
 }
 
+// Serving helper: the clouds of a batch copied one after another into `dst`
+// (an (N, 2) float32 device tensor) at the given row offsets, by one
+// asynchronous copy each on the current stream (host sources should be
+// pinned for the copies to overlap).  One call for the batch keeps the
+// per-request host cost to a copy submission.
+void copy_clouds(torch::Tensor dst, const std::vector<torch::Tensor>& srcs,
+                 const std::vector<int64_t>& rows) {
+  TORCH_CHECK(dst.is_cuda() && dst.is_contiguous() && dst.scalar_type() == torch::kFloat32 &&
+                  dst.dim() == 2 && dst.size(1) == 2,
+              "copy_clouds: dst must be a contiguous (N, 2) float32 ROCm tensor");
+  TORCH_CHECK(srcs.size() == rows.size(), "copy_clouds: one row offset per cloud");
+  c10::DeviceGuard guard(dst.device());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(dst.device().index()).stream();
+  float* base = dst.data_ptr<float>();
+  for (size_t i = 0; i < srcs.size(); ++i) {
+    const torch::Tensor& s = srcs[i];
+    TORCH_CHECK(s.is_contiguous() && s.scalar_type() == torch::kFloat32 && s.dim() == 2 &&
+                    s.size(1) == 2,
+                "copy_clouds: every cloud must be a contiguous (n, 2) float32 tensor");
+    TORCH_CHECK(rows[i] >= 0 && rows[i] + s.size(0) <= dst.size(0),
+                "copy_clouds: a cloud does not fit at its row offset");
+    if (s.size(0) == 0) continue;
+    C10_HIP_CHECK(hipMemcpyAsync(base + rows[i] * 2, s.data_ptr<float>(),
+                                 static_cast<size_t>(s.numel()) * sizeof(float),
+                                 s.is_cuda() ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                 stream));
+  }
+}
+
 // csrc/extension.cpp:65-81
 PYBIND11_MODULE(_C, m) {
   m.doc() = "Native PyTorch-ROCm DBSCAN (2D, grid-hashed) on gfx950.";
@@ -461,6 +490,10 @@ PYBIND11_MODULE(_C, m) {
         pybind11::arg("labels"), pybind11::arg("workspace"),
         "Serving helper: batched DBSCAN into caller-owned device buffers on the current "
         "stream; allocates nothing.");
+  m.def("copy_clouds", &copy_clouds, pybind11::arg("dst"), pybind11::arg("srcs"),
+        pybind11::arg("rows"),
+        "Serving helper: clouds copied into consecutive rows of a device tensor, one "
+        "asynchronous copy each on the current stream.");
   m.def("dbscan2d_batched_workspace_bytes", &dbscan2d_batched_workspace_bytes,
         pybind11::arg("total_points"), pybind11::arg("num_clouds"),
         "Workspace bytes of a batch of num_clouds clouds holding total_points points.");

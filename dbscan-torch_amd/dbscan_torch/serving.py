@@ -65,9 +65,10 @@ class Request:
 
 class _Slot:
     """One batch's share of the arena: points, labels, workspace and its
-    stream.  Each request's labels go device-to-host straight into a pinned
-    tensor of its own (torch's caching host allocator), so retiring a batch
-    copies nothing on the host."""
+    stream.  A batch's labels go device-to-host in one copy into a pinned
+    tensor of the batch's own (torch's caching host allocator); each request
+    receives its slice of it, so retiring a batch copies nothing on the host
+    (a request's labels keep that batch's buffer alive)."""
 
     def __init__(self, arena: torch.Tensor, offset: int, points: int, ws_bytes: int,
                  device: torch.device):
@@ -78,7 +79,8 @@ class _Slot:
         self.stream = torch.cuda.Stream(device)  # default priority
         self.event: Optional[torch.cuda.Event] = None
         self.batch: List[Request] = []
-        self.host_labels: List[torch.Tensor] = []
+        self.offsets: List[int] = []
+        self.host_labels: Optional[torch.Tensor] = None
 
     @staticmethod
     def nbytes(points: int, ws_bytes: int) -> int:
@@ -158,29 +160,28 @@ class Server:
             pts += r.n
             offs.append(pts)
         now = self.clock()
-        host = [torch.empty(r.n, dtype=torch.int32, pin_memory=True) for r in batch]
+        for r in batch:
+            r.admitted = now
+        host = torch.empty(pts, dtype=torch.int32, pin_memory=True)
         with torch.cuda.stream(slot.stream):
-            for r, o in zip(batch, offs):
-                r.admitted = now
-                slot.X[o:o + r.n].copy_(r.X, non_blocking=True)
+            _C.copy_clouds(slot.X, [r.X for r in batch], offs[:-1])
             _C.dbscan2d_batched_out(slot.X[:pts], offs, [r.eps for r in batch],
                                     [r.min_samples for r in batch], slot.labels,
                                     slot.workspace)
-            for h, r, o in zip(host, batch, offs):
-                h.copy_(slot.labels[o:o + r.n], non_blocking=True)
+            host.copy_(slot.labels[:pts], non_blocking=True)
             slot.event = torch.cuda.Event()
             slot.event.record(slot.stream)
-        slot.batch, slot.host_labels = batch, host
+        slot.batch, slot.offsets, slot.host_labels = batch, offs, host
         self.inflight.append(slot)
         self.batches.append(len(batch))
 
     def _retire(self, slot: _Slot) -> None:
         now = self.clock()
-        for r, h in zip(slot.batch, slot.host_labels):
-            r.labels = h
+        for r, o in zip(slot.batch, slot.offsets):
+            r.labels = slot.host_labels[o:o + r.n]
             r.done = now
             self.completed[r.rid] = r
-        slot.batch, slot.host_labels, slot.event = [], [], None
+        slot.batch, slot.offsets, slot.host_labels, slot.event = [], [], None, None
         self.free.append(slot)
 
     def step(self) -> bool:
