@@ -1702,18 +1702,42 @@ __device__ __forceinline__ int32_t lane_core_count(
     const int col = near_col(k);
     WinColumn w;
     load_column(cell_start, cell_flags, cell_box, row[2 * col], row[2 * col + 1], w);
+    // the column's cells nearest box first (the count stops at min_samples):
+    // cells within eps wholesale, then the partial ones by box distance
+    // (eps_sq may be +inf: the cells still to scan are a bit mask, not a
+    // distance sentinel)
+    float d[5];
+    uint32_t todo = 0;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      if (q >= w.cnt || total >= min_samples || w.j0 + q == c) continue;
-      const int32_t cs = w.start[q], ce = w.start[q + 1];
-      if (point_box_d2(x, y, w.box[q]) > eps_sq) continue;
-      if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq)) {
-        total += ce - cs;
-      } else if (ce - cs <= kLaneScan) {
+      d[q] = 0.f;
+      if (q >= w.cnt || w.j0 + q == c) continue;
+      d[q] = point_box_d2(x, y, w.box[q]);
+      if (d[q] > eps_sq) continue;
+      if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq))
+        total += w.start[q + 1] - w.start[q];
+      else
+        todo |= 1u << q;
+    }
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+      if (!todo || total >= min_samples) break;
+      int qm = -1;
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        if (((todo >> q) & 1u) && (qm < 0 || d[q] < d[qm])) qm = q;
+      todo &= ~(1u << qm);
+      int32_t cs = 0, ce = 0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        if (q == qm) {
+          cs = w.start[q];
+          ce = w.start[q + 1];
+        }
+      if (ce - cs <= kLaneScan)
         total = lane_count_range(xs, ys, cs, ce, x, y, eps_sq, min_samples, total);
-      } else {
+      else
         deferred = true;
-      }
     }
   }
   return total;
