@@ -2505,37 +2505,70 @@ __global__ __launch_bounds__(kThreads) void k_unpermute(
   for (uint32_t j = b0 + t; j < b1; j += kThreads) labels[j] = s_lab[j - b0];
 }
 
-// Border search for one point by one lane (dbscan_grid.hpp:419-445): window
-// cells in scan order (dx-major, ky ascending), skipping cells without a core
-// or with a box farther than eps; the first strict minimum of d2 over cores
-// with d2 <= eps_sq wins.  Sets `deferred` when a big cell would need a scan.
+// Border search for one point by one lane (dbscan_grid.hpp:419-445): over
+// the window cells holding a core with a box within eps, the first strict
+// minimum of d2 over cores with d2 <= eps_sq in the reference's scan order
+// wins.  Sets `deferred` when a big cell would need a scan.
 __device__ __forceinline__ int32_t lane_border(
     const float* __restrict__ xs, const float* __restrict__ ys,
     const uint8_t* __restrict__ is_core, float x, float y, const int32_t* __restrict__ row,
     const int32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const int32_t* __restrict__ cell_cluster, float eps_sq,
     float best_init, bool& deferred) {
+  // Cells are visited nearest first (columns from the point's own outwards,
+  // a column's cells by box distance), and a cell whose box is farther than
+  // the best d2 so far cannot hold a better or tying core, so the rest of
+  // the column is skipped.  The winner is the smallest (d2, scan index):
+  // the reference's first strict minimum in its scan order (dx-major, ky
+  // ascending), whatever the visiting order; all cores of a cell share its
+  // label.
   float best = best_init;
-  int32_t lab = kNoise;
-  for (int col = 0; col < 5; ++col) {
+  int32_t lab = kNoise, best_idx = 1 << 30;
+  for (int k = 0; k < 5; ++k) {
+    const int col = near_col(k);
     WinColumn w;
     load_column(cell_start, cell_flags, cell_box, row[2 * col], row[2 * col + 1], w);
+    float d[5];
+    uint32_t todo = 0;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
+      d[q] = 0.f;
       if (q >= w.cnt || !(w.flags[q] & kCellHasCore)) continue;
-      if (point_box_d2(x, y, w.box[q]) > eps_sq) continue;
-      const int32_t cs = w.start[q], ce = w.start[q + 1];
+      d[q] = point_box_d2(x, y, w.box[q]);
+      if (d[q] > eps_sq) continue;
+      todo |= 1u << q;
+    }
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+      if (!todo) break;
+      int qm = -1;
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        if (((todo >> q) & 1u) && (qm < 0 || d[q] < d[qm])) qm = q;
+      todo &= ~(1u << qm);
+      float dq = 0.f;
+      int32_t cs = 0, ce = 0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        if (q == qm) {
+          dq = d[q];
+          cs = w.start[q];
+          ce = w.start[q + 1];
+        }
+      if (dq > best) break;  // and every cell after it in this column
       if (ce - cs > kLaneScan) {
         deferred = true;
         continue;
       }
-      // the cell's smallest accepted d2 replaces the best only when strictly
-      // smaller: the sequential first-strict-minimum rule, since all cores of
-      // a cell share its label
       const uint32_t cm = lane_min_core_d2(xs, ys, is_core, cs, ce, x, y, eps_sq);
-      if (cm != 0xFFFFFFFFu && __uint_as_float(cm) < best) {
-        best = __uint_as_float(cm);
-        lab = cell_cluster[w.j0 + q];
+      const int32_t idx = col * 5 + qm;
+      if (cm != 0xFFFFFFFFu) {
+        const float v = __uint_as_float(cm);
+        if (v < best || (v == best && idx < best_idx)) {
+          best = v;
+          best_idx = idx;
+          lab = cell_cluster[w.j0 + qm];
+        }
       }
     }
   }
