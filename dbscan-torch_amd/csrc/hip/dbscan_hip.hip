@@ -2305,11 +2305,11 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
 // passes so that no label is written alone into a random cache line.
 // Buckets of B consecutive input positions (bucket_width(): at most
 // kMaxBuckets buckets of at most kUnpermMaxWidth positions, so inputs up to
-// 2^24 points; larger ones take k_label_direct): bucket b receives exactly the
-// points whose input position is in [b B, (b+1) B), so its pair region starts
-// at b B.  B is the smallest multiple of 256 (at least 4,096) that needs no
-// more than kMaxBuckets buckets: for 10M points 508 buckets of 80 KB of
-// labels, two k_unpermute workgroups per CU, one round over the chip.
+// 2^24 points; larger ones take k_label_direct): bucket b gathers the points
+// whose input position is in [b B, (b+1) B).  B is the smallest multiple of
+// 256 (at least 4,096) that needs no more than kMaxBuckets buckets: for 10M
+// points 508 buckets of 80 KB of labels, two k_unpermute workgroups per CU,
+// one round over the chip.
 constexpr int kMaxBuckets = 512;
 constexpr int kUnpermMaxWidth = 1 << 15;  // labels k_unpermute holds in LDS at once (128 KB)
 constexpr int kUnpermSmallWidth = 20480;  // up to here the 80 KB instantiation
@@ -2327,15 +2327,18 @@ struct LabelForm {
   using Slot = std::conditional_t<kPacked, uint32_t, uint64_t>;  // 64 KB of LDS either way
 };
 static_assert(kLabelThreads >= kMaxBuckets, "a thread per bucket");
-constexpr uint64_t kLabelTagAgg = 17ull << 48, kLabelTagPre = 18ull << 48;  // after the passes' tags
+static_assert(LabelForm<true>::kTile <= 1 << 16 && LabelForm<false>::kTile <= 1 << 16,
+              "a run's bounds within its tile");
+// A run of more pairs than this marks its bucket for the chunked unpermute.
+constexpr uint32_t kLongRun = 256;
 
 // Pass A: every sorted point's label (a core point: its cell's cluster id;
 // another point: the border label k_border left in sorted order) and input
-// position go, as a pair, to the point's bucket.  As in a radix pass, the
-// tile's pairs are grouped by bucket in LDS and each bucket's run lands after
-// the earlier tiles' runs of that bucket (decoupled look-back, one bucket per
-// thread), so the runs leave as contiguous writes (16 lanes per run); the
-// order inside a bucket is free.
+// position form a pair.  A tile groups its pairs by bucket in LDS and writes
+// them, in that order, to its own slice of the pair arrays (the tile's
+// sorted positions), one contiguous write; a table gives where each bucket's
+// run starts in the slice: row b, column tile, row nb holding the tile's
+// size.  No tile waits for another; the order inside a run is free.
 template <bool kPacked>
 __device__ __forceinline__ void label_tile(
     Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
@@ -2344,8 +2347,8 @@ __device__ __forceinline__ void label_tile(
     const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
     uint32_t* __restrict__ spare_xa, uint32_t* __restrict__ spare_xb,
     uint32_t* __restrict__ spare_ya, uint32_t* __restrict__ spare_yb,
-    uint64_t* __restrict__ status, int64_t n64, uint32_t B, uint32_t* s_cnt, uint32_t* s_len,
-    uint32_t* s_base, typename LabelForm<kPacked>::Slot* s_buf, uint32_t* s_scan) {
+    uint32_t* __restrict__ table, uint32_t* __restrict__ long_runs, int64_t n64, uint32_t B,
+    uint32_t* s_cnt, typename LabelForm<kPacked>::Slot* s_buf, uint32_t* s_scan) {
   using F = LabelForm<kPacked>;
   constexpr int T = kLabelThreads;
   constexpr int kLabelItems = F::kItems;
@@ -2381,20 +2384,15 @@ __device__ __forceinline__ void label_tile(
   const uint32_t cnt = owner ? s_cnt[t] : 0u;
   uint32_t ttot;
   const uint32_t lstart = block_excl_scan<T>(cnt, ttot, s_scan);
+  // run table: row b, column tile = where bucket b's run starts in the tile's
+  // slice; row nb = the tile's size (the end of the last run)
+  const uint32_t ntiles = (n + F::kTile - 1u) / F::kTile;
   if (owner) {
-    uint64_t excl = 0;
-    st_release(status + static_cast<uint64_t>(tile) * kMaxBuckets + t,
-               (tile == 0 ? kLabelTagPre : kLabelTagAgg) | cnt);
-    if (tile > 0) {
-      excl = digit_lookback(status, static_cast<int64_t>(tile), 0, t, kLabelTagAgg, kLabelTagPre,
-                            &prm->error, kMaxBuckets);
-      st_release(status + static_cast<uint64_t>(tile) * kMaxBuckets + t,
-                 kLabelTagPre | (excl + cnt));
-    }
-    s_base[t] = static_cast<uint32_t>(t) * B + static_cast<uint32_t>(excl);
-    s_len[t] = cnt;
+    table[static_cast<size_t>(t) * ntiles + tile] = lstart;
     s_cnt[t] = lstart;
+    if (cnt > kLongRun) long_runs[t] = 1u;
   }
+  if (t == 0) table[static_cast<size_t>(nb) * ntiles + tile] = tile_n;
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kLabelItems; ++k) {
@@ -2410,22 +2408,14 @@ __device__ __forceinline__ void label_tile(
     }
   }
   __syncthreads();
-  // each run by a group of 16 lanes
-  const uint32_t g = static_cast<uint32_t>(t) >> 4, gl = static_cast<uint32_t>(t) & 15u;
-  for (uint32_t b = g; b < nb; b += T / 16) {
-    const uint32_t len = s_len[b], ls = s_cnt[b], gs = s_base[b];
-    if (gs + len > min(n, (b + 1u) * B)) {
-      if (gl == 0) atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
-      continue;
-    }
-    for (uint32_t k = gl; k < len; k += 16) {
-      const typename F::Slot v = s_buf[ls + k];
-      if constexpr (kPacked) {
-        pair_i[gs + k] = v;
-      } else {
-        pair_i[gs + k] = static_cast<uint32_t>(v);
-        pair_l[gs + k] = static_cast<int32_t>(v >> 32);
-      }
+  // the tile's pairs, grouped by bucket, to the tile's own slice
+  for (uint32_t j = t; j < tile_n; j += T) {
+    const typename F::Slot v = s_buf[j];
+    if constexpr (kPacked) {
+      pair_i[base + j] = v;
+    } else {
+      pair_i[base + j] = static_cast<uint32_t>(v);
+      pair_l[base + j] = static_cast<int32_t>(v >> 32);
     }
   }
 }
@@ -2437,22 +2427,21 @@ __global__ __launch_bounds__(kLabelThreads) void k_label(
     const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
     uint32_t* __restrict__ spare_xa, uint32_t* __restrict__ spare_xb,
     uint32_t* __restrict__ spare_ya, uint32_t* __restrict__ spare_yb,
-    uint64_t* __restrict__ status, int64_t n64, uint32_t B) {
+    uint64_t* __restrict__ status, uint32_t* __restrict__ long_runs, int64_t n64, uint32_t B) {
   __shared__ uint32_t s_cnt[kMaxBuckets];  // counts, then the runs' local starts
-  __shared__ uint32_t s_len[kMaxBuckets];
-  __shared__ uint32_t s_base[kMaxBuckets];  // the runs' global starts
   __shared__ uint64_t s_raw[kLabelTile];    // the tile's pairs, 64 KB in either form
   __shared__ uint32_t s_scan[kLabelThreads / 64];
+  uint32_t* table = reinterpret_cast<uint32_t*>(status);
   if (prm->num_roots < kPackedRoots) {
     // the grid counts the two-word form's tiles; the packed form has half
     if (static_cast<int64_t>(blockIdx.x) * LabelForm<true>::kTile >= n64) return;
     label_tile<true>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, spare_ia, spare_ib,
-                     spare_xa, spare_xb, spare_ya, spare_yb, status, n64, B, s_cnt, s_len,
-                     s_base, reinterpret_cast<uint32_t*>(s_raw), s_scan);
+                     spare_xa, spare_xb, spare_ya, spare_yb, table, long_runs, n64, B, s_cnt,
+                     reinterpret_cast<uint32_t*>(s_raw), s_scan);
   } else {
     label_tile<false>(prm, idx_a, idx_b, point_cell, is_core, cell_cluster, spare_ia, spare_ib,
-                      spare_xa, spare_xb, spare_ya, spare_yb, status, n64, B, s_cnt, s_len,
-                      s_base, s_raw, s_scan);
+                      spare_xa, spare_xb, spare_ya, spare_yb, table, long_runs, n64, B, s_cnt,
+                      s_raw, s_scan);
   }
 }
 
@@ -2474,31 +2463,196 @@ __global__ __launch_bounds__(256) void k_label_direct(
     labels[dst] = is_core[i] ? cell_cluster[point_cell[i]] : lab_sorted[i];
 }
 
-// Pass B: one workgroup per bucket places the bucket's labels in LDS by input
-// position and writes them out in order (B <= kWidth).
+// Pass B: one workgroup per bucket reads the bucket's run in every tile's
+// slice (bounds from rows b and b+1 of the run table), places the labels in
+// LDS by input position and writes them out in order (B <= kWidth).  Runs
+// are mostly short (about B / tiles pairs), and then a group of 16 lanes
+// reads kUnpermRuns runs at a time, their loads in flight together; the
+// group's lane l holds, in registers, the bounds of the group's runs l,
+// l + 16, ...  But the radix sort is stable, so a tile inside a dense cell
+// holds its points in input order and gives a few buckets runs of hundreds
+// of pairs, which would leave one group doing most of such a bucket: the
+// label pass marks those buckets (a run of more than kLongRun pairs), and
+// for them the work is split into chunks of 64 pairs, chunk k of run i going
+// to wave (i + k) mod waves.  A wave then takes 64 runs at a time, one per
+// lane, and walks its chunks of them in order: the owner of its j-th chunk
+// is the last lane whose exclusive chunk count is <= j (a ballot), and
+// kUnpermRuns chunks' loads are in flight together.
+constexpr uint32_t kUnpermRuns = 8;
+static_assert(16 % kUnpermRuns == 0, "a step's runs share a bound register");
+
+template <bool kPacked>
+__device__ __forceinline__ void place_label(int32_t* s_lab, uint32_t v, int32_t w, uint32_t b0,
+                                            uint32_t b1) {
+  if constexpr (kPacked) {
+    const uint32_t d = b0 + (v & 0x7FFFu);
+    if (d < b1) s_lab[d - b0] = static_cast<int32_t>(v >> 15) - 1;
+  } else {
+    if (v >= b0 && v < b1) s_lab[v - b0] = w;
+  }
+}
+
+// kUnpermRuns runs of a 16-lane group: the group's runs r0 .. r0 + 7, tiles
+// g + G r; lo / hi hold the bounds of run (r0 & ~15) + lane-in-group.
+template <bool kPacked>
+__device__ __forceinline__ void unpermute_runs(const uint32_t* __restrict__ pair_i,
+                                               const int32_t* __restrict__ pair_l, int32_t* s_lab,
+                                               uint32_t lo, uint32_t hi, uint32_t r0, uint32_t g,
+                                               uint32_t gl, uint32_t G, uint32_t ntiles,
+                                               uint32_t tile_sz, uint32_t n, uint32_t b0,
+                                               uint32_t b1) {
+  uint32_t s[kUnpermRuns], e[kUnpermRuns], off[kUnpermRuns];
+#pragma unroll
+  for (uint32_t u = 0; u < kUnpermRuns; ++u) {
+    const uint32_t r = (r0 & 15u) + u;
+    s[u] = __shfl(lo, static_cast<int>(r), 16);
+    e[u] = __shfl(hi, static_cast<int>(r), 16);
+    const uint32_t i = g + G * (r0 + u);
+    off[u] = i * tile_sz;
+    e[u] = i >= ntiles ? 0u : min(e[u], n - off[u]);
+  }
+  for (uint32_t c = gl;; c += 16u) {
+    bool any = false;
+    uint32_t v[kUnpermRuns];
+    int32_t w[kUnpermRuns];
+#pragma unroll
+    for (uint32_t u = 0; u < kUnpermRuns; ++u) {
+      if (s[u] + c < e[u]) {
+        any = true;
+        v[u] = pair_i[off[u] + s[u] + c];
+        if constexpr (!kPacked) w[u] = pair_l[off[u] + s[u] + c];
+      }
+    }
+    if (!any) break;
+#pragma unroll
+    for (uint32_t u = 0; u < kUnpermRuns; ++u)
+      if (s[u] + c < e[u]) place_label<kPacked>(s_lab, v[u], w[u], b0, b1);
+  }
+}
+
+// The chunks a wave takes of 64 runs (one per lane: its start, length, first
+// chunk k0 for this wave and exclusive count of such chunks over the lanes).
+template <bool kPacked>
+__device__ __forceinline__ void unpermute_chunks(const uint32_t* __restrict__ pair_i,
+                                                 const int32_t* __restrict__ pair_l,
+                                                 int32_t* s_lab, uint32_t excl, uint32_t k0,
+                                                 uint32_t start, uint32_t len, uint32_t total,
+                                                 uint32_t lane, uint32_t W, uint32_t b0,
+                                                 uint32_t b1) {
+  for (uint32_t j = 0; j < total; j += kUnpermRuns) {
+    uint32_t v[kUnpermRuns];
+    int32_t w[kUnpermRuns];
+    bool ok[kUnpermRuns];
+#pragma unroll
+    for (uint32_t u = 0; u < kUnpermRuns; ++u) {
+      ok[u] = false;
+      const uint32_t jj = j + u;
+      if (jj >= total) continue;  // the same for the whole wave
+      const uint64_t m = __ballot(excl <= jj);
+      const int owner = 63 - __builtin_clzll(m);
+      const uint32_t e_o = __builtin_amdgcn_readlane(excl, owner);
+      const uint32_t k_o = __builtin_amdgcn_readlane(k0, owner);
+      const uint32_t s_o = __builtin_amdgcn_readlane(start, owner);
+      const uint32_t l_o = __builtin_amdgcn_readlane(len, owner);
+      const uint32_t off = (k_o + (jj - e_o) * W) * 64u + lane;
+      if (off < l_o) {
+        ok[u] = true;
+        v[u] = pair_i[s_o + off];
+        if constexpr (!kPacked) w[u] = pair_l[s_o + off];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kUnpermRuns; ++u)
+      if (ok[u]) place_label<kPacked>(s_lab, v[u], w[u], b0, b1);
+  }
+}
+
 template <int kWidth, int kThreads>
 __global__ __launch_bounds__(kThreads) void k_unpermute(
     const Params* __restrict__ prm, const uint32_t* __restrict__ spare_xa,
     const uint32_t* __restrict__ spare_xb, const uint32_t* __restrict__ spare_ya,
-    const uint32_t* __restrict__ spare_yb, int32_t* __restrict__ labels, int64_t n64,
+    const uint32_t* __restrict__ spare_yb, const uint32_t* __restrict__ table,
+    const uint32_t* __restrict__ long_runs, int32_t* __restrict__ labels, int64_t n64,
     uint32_t B) {
+  constexpr uint32_t W = kThreads / 64;  // waves
+  constexpr uint32_t G = kThreads / 16;  // groups; group g reads the runs of tiles g, g + G, ...
+  static_assert((W & (W - 1)) == 0, "waves per workgroup a power of two");
+  // bound registers for the most tiles an input of this width has in the
+  // two-word form (more tiles take the chunked path)
+  constexpr uint32_t kMaxTiles =
+      (static_cast<uint32_t>(kMaxBuckets) * kWidth + LabelForm<false>::kTile - 1) /
+      LabelForm<false>::kTile;
+  constexpr uint32_t kRL = (kMaxTiles + 16 * G - 1) / (16 * G);
   __shared__ int32_t s_lab[kWidth];
   const bool packed = prm->num_roots < kPackedRoots;
   const uint32_t n = static_cast<uint32_t>(n64);
+  const uint32_t tile_sz = packed ? LabelForm<true>::kTile : LabelForm<false>::kTile;
+  const uint32_t ntiles = (n + tile_sz - 1u) / tile_sz;
   const bool in_a = (prm->npasses & 1) != 0;
   const uint32_t* __restrict__ pair_i = in_a ? spare_xb : spare_xa;
   const int32_t* __restrict__ pair_l = reinterpret_cast<const int32_t*>(in_a ? spare_yb : spare_ya);
-  const int t = threadIdx.x;
-  const uint32_t b0 = static_cast<uint32_t>(blockIdx.x) * B;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  // Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous range of
+  // buckets, so a cache line holding the ends of two neighbouring runs of a
+  // tile is read into one XCD's L2, not two.
+  const uint32_t nbk = gridDim.x, x = blockIdx.x & 7u, q8 = nbk >> 3, r8 = nbk & 7u;
+  const uint32_t b = x * q8 + min(x, r8) + (blockIdx.x >> 3);
+  const uint32_t b0 = b * B;
   const uint32_t b1 = min(n, b0 + min(B, static_cast<uint32_t>(kWidth)));
-  for (uint32_t j = b0 + t; j < b1; j += kThreads) {
-    if (packed) {
-      const uint32_t v = pair_i[j];
-      const uint32_t d = b0 + (v & 0x7FFFu);
-      if (d < b1) s_lab[d - b0] = static_cast<int32_t>(v >> 15) - 1;
-    } else {
-      const uint32_t d = pair_i[j];
-      if (d >= b0 && d < b1) s_lab[d - b0] = pair_l[j];
+  const uint32_t* __restrict__ lo_row = table + static_cast<size_t>(b) * ntiles;
+  const uint32_t* __restrict__ hi_row = lo_row + ntiles;
+  if (long_runs[b] != 0u || ntiles > 16u * G * kRL) {
+    for (uint32_t base = 0; base < ntiles; base += 64u) {
+      const uint32_t i = base + lane;
+      uint32_t start = 0u, len = 0u;
+      if (i < ntiles) {
+        const uint32_t lo = lo_row[i];
+        const uint32_t hi = min(min(hi_row[i], tile_sz), n - i * tile_sz);
+        start = i * tile_sz + lo;
+        len = hi > lo ? hi - lo : 0u;
+      }
+      const uint32_t nch = (len + 63u) >> 6;
+      const uint32_t k0 = (wv - i) & (W - 1u);  // the first chunk of run i this wave takes
+      const uint32_t cnt = k0 < nch ? (nch - k0 + W - 1u) / W : 0u;
+      uint32_t incl = cnt;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= static_cast<uint32_t>(d)) incl += o;
+      }
+      const uint32_t total = __shfl(incl, 63, 64);
+      if (packed)
+        unpermute_chunks<true>(pair_i, pair_l, s_lab, incl - cnt, k0, start, len, total, lane, W,
+                               b0, b1);
+      else
+        unpermute_chunks<false>(pair_i, pair_l, s_lab, incl - cnt, k0, start, len, total, lane,
+                                W, b0, b1);
+    }
+  } else {
+    const uint32_t g = t >> 4, gl = t & 15u;
+    const uint32_t g_wave = wv * 4u;  // the wave's first group
+    uint32_t lo[kRL], hi[kRL];
+#pragma unroll
+    for (uint32_t q = 0; q < kRL; ++q) {
+      const uint32_t i = g + G * (gl + 16u * q);
+      lo[q] = hi[q] = 0u;
+      if (i < ntiles) {
+        lo[q] = lo_row[i];
+        hi[q] = min(hi_row[i], tile_sz);
+      }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < kRL; ++q) {
+      for (uint32_t r = 0; r < 16u; r += kUnpermRuns) {
+        const uint32_t r0 = 16u * q + r;
+        if (g_wave + G * r0 >= ntiles) break;  // the same for the whole wave
+        if (packed)
+          unpermute_runs<true>(pair_i, pair_l, s_lab, lo[q], hi[q], r0, g, gl, G, ntiles, tile_sz,
+                               n, b0, b1);
+        else
+          unpermute_runs<false>(pair_i, pair_l, s_lab, lo[q], hi[q], r0, g, gl, G, ntiles,
+                                tile_sz, n, b0, b1);
+      }
     }
   }
   __syncthreads();
@@ -2724,7 +2878,7 @@ __global__ __launch_bounds__(256) void k_border(
 // Workspace layout for n points.  clouds > 0: a batch of that many clouds
 // holding n points in all (tile counts are bounded by n / tile + clouds).
 struct Layout {
-  size_t params, sort_status, cell_status, rank_status, cloud_hist, zero_end;
+  size_t params, sort_status, cell_status, rank_status, cloud_hist, long_runs, zero_end;
   size_t partials, clouds, seg_sort, seg_scan, seg_end;
   size_t xs_a, ys_a, idx_a, xs_b, ys_b, idx_b;
   size_t point_cell, is_core, cell_start, cell_key, cell_flags, cell_box, ngh, cell_ref, uf, rank,
@@ -2748,13 +2902,15 @@ Layout make_layout(int64_t n, int64_t clouds = 0) {
     return at;
   };
   L.params = take(sizeof(Params));
-  // the radix passes' look-back words, reused by the label pass (k_label)
+  // the radix passes' look-back words, reused for the label pass's run table
+  // ((kMaxBuckets + 1) x tiles words)
   const size_t label_tiles = (N + kLabelTile - 1) / kLabelTile;
   L.sort_status = take(std::max(sort_tiles * kRadixMax, label_tiles * kMaxBuckets) *
                        sizeof(uint64_t));
   L.cell_status = take(scan_tiles * sizeof(uint64_t));
   L.rank_status = take(scan_tiles * sizeof(uint64_t));
   L.cloud_hist = take(B * kMaxPasses * kRadixMax * sizeof(uint32_t));
+  L.long_runs = take(kMaxBuckets * sizeof(uint32_t));
   L.zero_end = off;
   L.partials = take(kMaxReduceBlocks * sizeof(int4));
   L.clouds = take(B * sizeof(CloudInfo));
@@ -2801,15 +2957,17 @@ uint32_t bucket_width(int64_t n) {
 }
 
 void launch_unpermute(uint32_t B, int64_t n, hipStream_t stream, const Params* prm,
-                      float* xs_a, float* xs_b, float* ys_a, float* ys_b, int32_t* labels) {
+                      float* xs_a, float* xs_b, float* ys_a, float* ys_b, const uint64_t* status,
+                      const uint32_t* long_runs, int32_t* labels) {
   const int nb = static_cast<int>((n + B - 1) / B);
   auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
+  const uint32_t* table = reinterpret_cast<const uint32_t*>(status);
   if (B <= static_cast<uint32_t>(kUnpermSmallWidth))
-    k_unpermute<kUnpermSmallWidth, 512><<<nb, 512, 0, stream>>>(prm, u(xs_a), u(xs_b), u(ys_a),
-                                                                 u(ys_b), labels, n, B);
+    k_unpermute<kUnpermSmallWidth, 512><<<nb, 512, 0, stream>>>(
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), table, long_runs, labels, n, B);
   else
-    k_unpermute<kUnpermMaxWidth, 1024><<<nb, 1024, 0, stream>>>(prm, u(xs_a), u(xs_b), u(ys_a),
-                                                                 u(ys_b), labels, n, B);
+    k_unpermute<kUnpermMaxWidth, 1024><<<nb, 1024, 0, stream>>>(
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), table, long_runs, labels, n, B);
 }
 
 int grid_for(int64_t items, int per_block, int cap);
@@ -2818,11 +2976,11 @@ int grid_for(int64_t items, int per_block, int cap);
 void launch_label(uint32_t B, int64_t n, hipStream_t stream, Params* prm, const uint32_t* idx_a,
                   const uint32_t* idx_b, const int32_t* point_cell, const uint8_t* is_core,
                   const int32_t* cell_cluster, float* xs_a, float* xs_b, float* ys_a,
-                  float* ys_b, uint64_t* status) {
+                  float* ys_b, uint64_t* status, uint32_t* long_runs) {
   auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
   k_label<<<grid_for(n, kLabelTile, INT_MAX), kLabelThreads, 0, stream>>>(
       prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, u(xs_a), u(xs_b),
-      u(ys_a), u(ys_b), status, n, B);
+      u(ys_a), u(ys_b), status, long_runs, n, B);
 }
 
 int grid_for(int64_t items, int per_block, int cap) {
@@ -3106,13 +3264,15 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   const uint32_t bw = bucket_width(n);
   if (bw > 0) {
     launch_label(bw, n, stream, prm, idx_a, idx_b, point_cell, is_core, cell_cluster, xs_a, xs_b,
-                 ys_a, ys_b, sort_status);
+                 ys_a, ys_b, sort_status, at<uint32_t>(ws, L.long_runs));
   } else {
     k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
         prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
   }
   if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
-  if (bw > 0) launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, labels);
+  if (bw > 0)
+    launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, sort_status,
+                     at<uint32_t>(ws, L.long_runs), labels);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
@@ -3320,13 +3480,15 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const uint32_t bw = bucket_width(n);
   if (bw > 0) {
     launch_label(bw, n, stream, prm, idx_a, idx_b, point_cell, is_core, cell_cluster, xs_a, xs_b,
-                 ys_a, ys_b, sort_status);
+                 ys_a, ys_b, sort_status, at<uint32_t>(ws, L.long_runs));
   } else {
     k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
         prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
   }
   if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
-  if (bw > 0) launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, labels);
+  if (bw > 0)
+    launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, sort_status,
+                     at<uint32_t>(ws, L.long_runs), labels);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   return DBSCAN_HIP_OK;

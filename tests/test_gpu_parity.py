@@ -131,6 +131,24 @@ def test_label_pair_forms(dt, clusters):
     assert np.array_equal(labels, oracle.dbscan2d(X, 0.5, 1))
 
 
+@pytest.mark.parametrize("clusters", [100_000, 300_000])
+def test_dense_cell_long_runs(dt, clusters):
+    """A cell of 200,000 points sorts in input order, so the label pass's
+    tiles inside it hold runs of hundreds of pairs for a few buckets; those
+    buckets are unpermuted in 64-pair chunks shared by every wave.  Isolated
+    points (one cluster each, both pair forms) plus the dense cell, shuffled,
+    against the oracle."""
+    side = int(np.ceil(np.sqrt(clusters)))
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)[:clusters]
+    rng = np.random.default_rng(clusters + 1)
+    dense = rng.uniform(-10.1, -10.0, (200_000, 2))
+    X = np.concatenate([g, dense]).astype(np.float32)
+    X = X[rng.permutation(len(X))]
+    labels = gpu_labels(dt, X, 0.5, 1)
+    assert int(labels.max()) + 1 == clusters + 1
+    assert np.array_equal(labels, oracle.dbscan2d(X, 0.5, 1))
+
+
 def test_back_to_back_calls_reuse_the_workspace(dt):
     """Calls that reuse a workspace must not see the previous call's look-back
     words (a stale inclusive prefix once gave the last scan tile prefix 0):
