@@ -53,10 +53,11 @@ constexpr int32_t kNoise = -1;
 // Radix digits are at most r bits wide (r = the configuration's radix_bits,
 // at most kRadixBitsMax).  A key of b bits is sorted in P = min(ceil(b / r),
 // kMaxPasses) passes of ceil(b / P) bits each, both chosen on the device once
-// the bounds are known (26-bit keys, r = 8: 4 passes of 7 bits).  In a batch
+// the bounds are known (r = 9: 25-bit keys take 3 passes of 9 bits, 28-bit
+// keys 4 passes of 7 bits).  In a batch
 // every cloud takes the batch's largest P and splits its own key bits over
 // those P passes.
-constexpr int kRadixBitsMax = 8;
+constexpr int kRadixBitsMax = 9;
 constexpr int kRadixMax = 1 << kRadixBitsMax;
 constexpr int kMaxPasses = 8;           // 64-bit keys at most
 constexpr int kSortThreads = 512;
@@ -736,10 +737,18 @@ __device__ __forceinline__ void sort_pass_tiles(
     __shared__ uint32_t s_wcnt[kSortThreads / 64][kRadixMax];
     __shared__ uint32_t s_goff[kRadixMax];
     __shared__ uint32_t s_dstart[kRadixMax];
-    __shared__ uint32_t s_scan[kSortThreads / 64];
+    // the two block scans finish (a barrier follows) before s_d is first
+    // written, and s_d is last read before the tile's closing barrier: the
+    // scan words live in its first bytes (12-point tiles of 9-bit digits then
+    // fit 80 KB, two workgroups per CU)
+    uint32_t* s_scan = reinterpret_cast<uint32_t*>(s_d);
+    static_assert(kTile * sizeof(uint16_t) >= (kSortThreads / 64) * sizeof(uint32_t), "");
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int e = t; e < (kSortThreads / 64) * kRadixMax; e += kSortThreads) (&s_wcnt[0][0])[e] = 0u;
+    // the pass's digits: only these are counted, published and looked back on
+    const int nd = 1 << m.dbits;
+    for (int e = t; e < (kSortThreads / 64) * nd; e += kSortThreads)
+      s_wcnt[e >> m.dbits][e & (nd - 1)] = 0u;
     // global start of each owned digit: exclusive scan of the pass histogram
     uint32_t goff[D];
     {
@@ -813,7 +822,7 @@ __device__ __forceinline__ void sort_pass_tiles(
     for (int q = 0; q < D; ++q) {
       const int d = t * D + q;
       dtot[q] = 0;
-      if (d < kRadixMax) {
+      if (d < nd) {
         // per-wave counts -> exclusive offsets of the waves
 #pragma unroll
         for (int w = 0; w < kSortThreads / 64; ++w) {
@@ -831,7 +840,7 @@ __device__ __forceinline__ void sort_pass_tiles(
 #pragma unroll
     for (int q = 0; q < D; ++q) {
       const int d = t * D + q;
-      if (d >= kRadixMax) break;
+      if (d >= nd) break;
       uint64_t excl = 0;
       if (tile > first) {
         excl = digit_lookback(status, static_cast<int64_t>(tile), static_cast<int64_t>(first), d,
@@ -893,9 +902,11 @@ __device__ __forceinline__ void sort_pass_tiles(
   }
 }
 
-// One radix pass, one tile per workgroup.
+// One radix pass, one tile per workgroup.  Tiles of up to 12 points per
+// thread fit two workgroups per CU in LDS; the register budget is held to
+// match (4 waves per SIMD: at most 128 VGPRs).
 template <bool kFirst, bool kSeg, int kItems>
-__global__ __launch_bounds__(kSortThreads) void k_sort_pass(
+__global__ __launch_bounds__(kSortThreads, kItems <= 12 ? 4 : 2) void k_sort_pass(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
@@ -3033,11 +3044,14 @@ constexpr int kStageSlots[DBSCAN_HIP_NUM_STAGES][2] = {
 
 // Built-in launch configuration by input size: the autotuner's table for
 // MI355X (dbscan_torch/tuning_gfx950.json); a caller can pass its own.
+// (Widening radix_bits changes nothing unless it saves a pass: the digit
+// width is the key width over the pass count.  At 10M points, 9 bits sort
+// 25-27-bit keys in 3 passes of about 94 us instead of 4 of about 74 us.)
 dbscan_hip_config default_config(int64_t n) {
-  if (n <= 31622) return dbscan_hip_config{8, 7, 2048, 16384};
+  if (n <= 31622) return dbscan_hip_config{8, 8, 2048, 16384};
   if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 4096};
-  if (n <= 3162277) return dbscan_hip_config{8, 8, 2048, 65536};
-  return dbscan_hip_config{12, 8, 8192, 8192};
+  if (n <= 3162277) return dbscan_hip_config{8, 7, 8192, 8192};
+  return dbscan_hip_config{12, 9, 8192, 8192};
 }
 
 bool config_ok(const dbscan_hip_config& c) {
@@ -3046,8 +3060,8 @@ bool config_ok(const dbscan_hip_config& c) {
          c.pair_blocks >= 64 && c.pair_blocks <= 65536;
 }
 
-// Passes from kTailPass on are needed only for keys wider than 32 bits
-// (4 x 8-bit digits); they run in one launch (k_sort_tail) of one workgroup
+// Passes from kTailPass on are needed only for keys wider than 4 digits of
+// the configuration's radix_bits (36 bits at 9); they run in one launch (k_sort_tail) of one workgroup
 // per CU that walks the tiles, so that for the usual narrow key they cost a
 // single launch that returns at once.  One workgroup per CU is always
 // co-resident, which the walk and the barrier between passes need: a

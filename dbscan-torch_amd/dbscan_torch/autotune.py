@@ -38,7 +38,7 @@ TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_gf
 # representative size of each bucket; a bucket covers sizes up to the
 # geometric midpoint with the next one
 BUCKETS = (10_000, 100_000, 1_000_000, 10_000_000)
-SORT_SHAPES = [(items, bits) for items in (8, 12, 16) for bits in (6, 7, 8)]
+SORT_SHAPES = [(items, bits) for items in (8, 12, 16) for bits in (7, 8, 9)]
 CELL_BLOCKS = (1024, 2048, 4096, 8192, 16384, 32768)
 PAIR_BLOCKS = (2048, 4096, 8192, 16384, 32768, 65536)
 INT32_MAX = 2**31 - 1

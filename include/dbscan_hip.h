@@ -46,7 +46,7 @@ enum {
  */
 typedef struct {
   int32_t sort_items;   /* points per thread of a radix-sort tile (512 threads): 8, 12 or 16 */
-  int32_t radix_bits;   /* widest radix digit used when it saves no pass: 4..8   */
+  int32_t radix_bits;   /* widest radix digit used when it saves no pass: 4..9   */
   int32_t cell_blocks;  /* workgroups (of 256) of the per-cell kernels: 64..65536 */
   int32_t pair_blocks;  /* workgroups of the cell-pair union kernels: 64..65536  */
 } dbscan_hip_config;

@@ -194,8 +194,8 @@ def test_launch_configuration_checks():
     lib.dbscan2d_hip_default_config.argtypes = [ctypes.c_int64, ctypes.POINTER(_Cfg)]
     d = _Cfg()
     assert lib.dbscan2d_hip_default_config(10_000_000, ctypes.byref(d)) == 0
-    assert d.sort_items in (8, 12, 16) and 4 <= d.radix_bits <= 8
-    for bad in ((10, 8, 8192, 8192), (12, 9, 8192, 8192), (12, 3, 8192, 8192),
+    assert d.sort_items in (8, 12, 16) and 4 <= d.radix_bits <= 9
+    for bad in ((10, 8, 8192, 8192), (12, 10, 8192, 8192), (12, 3, 8192, 8192),
                 (12, 8, 10, 8192), (12, 8, 8192, 1 << 20)):
         c = _Cfg(*bad)
         # fake, aligned addresses: the configuration is refused before any device work
