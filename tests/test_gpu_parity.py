@@ -188,6 +188,34 @@ def test_wide_keys_take_the_tail_passes(dt):
     assert np.array_equal(out[0].cpu().numpy(), oracle.dbscan2d(small, 0.5, 3))
 
 
+def _key_bits(X, eps):
+    """Bits of the compressed cell key of X: bit widths of the kx and ky
+    ranges at cell side eps / sqrt(2), as the device computes them."""
+    inv = np.float32(1.0) / np.float32(np.float32(eps) / np.float32(np.sqrt(2.0)))
+    k = np.floor(X.astype(np.float32) * inv).astype(np.int64)
+    return sum(int(k[:, d].max() - k[:, d].min()).bit_length() for d in (0, 1))
+
+
+@pytest.mark.parametrize("bits", [18, 27])
+def test_nine_bit_digits(dt, bits):
+    """Radix digits of 9 bits: an 18-bit key sorts in 2 passes and a 27-bit key
+    in 3 when radix_bits is 9 (6-bit digits at radix_bits 8, 7-bit at 7), with
+    every tile shape, against the oracle."""
+    if bits == 18:
+        X = W.G(300_000, 60, (0, 140), 0.7, 0.05, (0, 140), 12)
+    else:
+        X = W.G(1_000_000, 300, (0, 4000), 1.0, 0.1, (0, 4000), 11)
+        X[:, 1] *= np.float32(0.5)
+    X = np.clip(X, 0, None).astype(np.float32)
+    assert _key_bits(X, 0.5) == bits
+    ref = oracle.dbscan2d(X, 0.5, 10)
+    Xd = torch.from_numpy(X).to(DEV)
+    for cfg in ((12, 9, 8192, 8192), (8, 9, 2048, 16384), (16, 9, 64, 64), (12, 7, 8192, 8192)):
+        print(f"[gpu] {bits}-bit key, config {cfg}", flush=True)
+        got = dt._C.dbscan2d_configured(Xd, 0.5, 10, list(cfg))
+        assert np.array_equal(got.cpu().numpy(), ref), cfg
+
+
 def test_permutation_equivariance_lattice1m(dt):
     X, eps, ms = W.lattice1m()
     base = gpu_labels(dt, X, eps, ms)
@@ -301,7 +329,7 @@ def test_eight_byte_aligned_input(dt):
     assert np.array_equal(dt.dbscan2d(view, eps, ms).cpu().numpy(), oracle.dbscan2d(X, eps, ms))
 
 
-_CONFIGS = [(i, b, cb, pb) for i in (8, 12, 16) for b in (6, 7, 8)
+_CONFIGS = [(i, b, cb, pb) for i in (8, 12, 16) for b in (6, 7, 8, 9)
             for cb, pb in ((64, 64), (65536, 65536))]
 
 
