@@ -130,7 +130,8 @@ struct KeyMap {
   uint32_t kxmin, kymin;  // subtracted (mod 2^32) from kx, ky
   uint32_t bx, by;        // bit widths of the x and y parts
   int npasses;            // radix passes covering bx + by bits (>= 1)
-  uint32_t dbits;         // digit width of every pass
+  uint32_t dbits;         // digit width of passes 1, 2, ...
+  uint32_t dlow;          // digit width of pass 0: the key bits the others leave
 };
 
 __device__ __forceinline__ uint32_t bit_width(uint32_t v) {
@@ -153,6 +154,12 @@ __device__ __forceinline__ KeyMap make_keymap(int32_t kxmin, int32_t kxmax, int3
   m.npasses = passes > 0 ? passes
                          : (nbits ? min((nbits + rbits - 1) / rbits, kMaxPasses) : 1);
   m.dbits = nbits ? static_cast<uint32_t>((nbits + m.npasses - 1) / m.npasses) : 1u;
+  // The passes' spare bits go to pass 0 (a 26-bit key in 3 passes: 8 + 9 + 9):
+  // its digits, the lowest key bits, are spread the most evenly over a tile,
+  // so fewer and longer runs help it most.  With fewer key bits than passes
+  // every pass keeps dbits and the top digits are 0.
+  const int rest = (m.npasses - 1) * static_cast<int>(m.dbits);
+  m.dlow = nbits > rest ? static_cast<uint32_t>(nbits - rest) : m.dbits;
   return m;
 }
 
@@ -227,10 +234,14 @@ __device__ __forceinline__ uint64_t sort_key(float x, float y, float inv_r, cons
   return (static_cast<uint64_t>(xr) << m.by) | yr;
 }
 
-// Digit of radix pass `pass`.
+// Width and digit of radix pass `pass`: pass 0 takes the lowest dlow bits,
+// pass p > 0 the dbits above dlow + (p - 1) dbits.
+__device__ __forceinline__ uint32_t digit_width(int pass, const KeyMap& m) {
+  return pass == 0 ? m.dlow : m.dbits;
+}
 __device__ __forceinline__ uint32_t key_digit(uint64_t key, int pass, const KeyMap& m) {
-  return static_cast<uint32_t>(key >> (m.dbits * static_cast<uint32_t>(pass))) &
-         ((1u << m.dbits) - 1u);
+  const uint32_t shift = pass == 0 ? 0u : m.dlow + m.dbits * static_cast<uint32_t>(pass - 1);
+  return static_cast<uint32_t>(key >> shift) & ((1u << digit_width(pass, m)) - 1u);
 }
 
 // Cell key in internal order: (x' << 32 | y').
@@ -746,9 +757,10 @@ __device__ __forceinline__ void sort_pass_tiles(
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     // the pass's digits: only these are counted, published and looked back on
-    const int nd = 1 << m.dbits;
+    const uint32_t dw = digit_width(pass, m);
+    const int nd = 1 << dw;
     for (int e = t; e < (kSortThreads / 64) * nd; e += kSortThreads)
-      s_wcnt[e >> m.dbits][e & (nd - 1)] = 0u;
+      s_wcnt[e >> dw][e & (nd - 1)] = 0u;
     // global start of each owned digit: exclusive scan of the pass histogram
     uint32_t goff[D];
     {
@@ -800,7 +812,7 @@ __device__ __forceinline__ void sort_pass_tiles(
       uint64_t peers = __ballot(valid);
 #pragma unroll
       for (int b = 0; b < kRadixBitsMax; ++b) {
-        if (static_cast<uint32_t>(b) < m.dbits) {
+        if (static_cast<uint32_t>(b) < dw) {
           const bool bit = (dg[k] >> b) & 1u;
           const uint64_t bal = __ballot(bit);
           peers &= bit ? bal : ~bal;
