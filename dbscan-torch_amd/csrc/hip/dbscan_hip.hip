@@ -849,6 +849,28 @@ __device__ __forceinline__ void sort_pass_tiles(
     }
     uint32_t ttot;
     uint32_t dstart = block_excl_scan<kSortThreads>(dsum, ttot, s_scan);
+    uint32_t dst0[D];  // tile-local start of each owned digit
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const int d = t * D + q;
+      dst0[q] = dstart;
+      if (d < nd) s_dstart[d] = dstart;
+      dstart += dtot[q];
+    }
+    __syncthreads();
+
+    // Tile-local position of every item (kept in rk); the digits, and (x, y),
+    // go to LDS in position order before the look-back, so that x, y and the
+    // digits are no longer held in registers while it waits.
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
+      if (wl + k * 64 < tile_n) {
+        s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
+        s_buf[rk[k]] = (static_cast<uint64_t>(__float_as_uint(y[k])) << 32) | __float_as_uint(x[k]);
+      }
+    }
+    // Global start of each owned digit: the earlier tiles' counts by look-back.
 #pragma unroll
     for (int q = 0; q < D; ++q) {
       const int d = t * D + q;
@@ -859,28 +881,12 @@ __device__ __forceinline__ void sort_pass_tiles(
                               tag_agg, tag_pre, &prm->error);
         st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
       }
-      s_dstart[d] = dstart;
-      s_goff[d] = goff[q] + static_cast<uint32_t>(excl) - dstart;
-      dstart += dtot[q];
+      s_goff[d] = goff[q] + static_cast<uint32_t>(excl) - dst0[q];
     }
     __syncthreads();
-
-    // Tile-local position of every item (kept in rk), then the digits in
-    // position order.
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      rk[k] += s_dstart[dg[k]] + s_wcnt[wave][dg[k]];
-      if (wl + k * 64 < tile_n) s_d[rk[k]] = static_cast<uint16_t>(dg[k]);
-    }
     // The output position of each element this thread writes (element j = t +
-    // kSortThreads q of the tile, in digit order), then (x, y) and the index through the
-    // one LDS buffer.
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      if (wl + k * 64 < tile_n)
-        s_buf[rk[k]] = (static_cast<uint64_t>(__float_as_uint(y[k])) << 32) | __float_as_uint(x[k]);
-    }
-    __syncthreads();
+    // kSortThreads q of the tile, in digit order), then (x, y) and the index
+    // through the one LDS buffer.
     uint32_t pos[kItems];
 #pragma unroll
     for (int q = 0; q < kItems; ++q) {
