@@ -1037,6 +1037,13 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
   const uint32_t e0 = base + static_cast<uint32_t>(t) * kCellItems;
   const bool whole = e0 + kCellItems <= end;
 
+  // thread 0's previous point (the last of the tile before), loaded with the
+  // tile's own points rather than after them
+  float bx = 0.f, by = 0.f;
+  if (t == 0 && base > cbeg) {
+    bx = xs[base - 1];
+    by = ys[base - 1];
+  }
   float px[kCellItems], py[kCellItems];
   if (whole && (e0 & 3u) == 0) {
 #pragma unroll
@@ -1070,7 +1077,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cells(
     have_prev = true;
   } else if (base > cbeg) {
     uint32_t xr, yr;
-    rel_cell(xs[base - 1], ys[base - 1], inv_r, m, xr, yr);
+    rel_cell(bx, by, inv_r, m, xr, yr);
     prev = cell_key64(xr, yr);
     have_prev = true;
   }
@@ -2277,13 +2284,20 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
   const uint32_t tile = blockIdx.x;  // see kTileOrder
   const int64_t base = static_cast<int64_t>(tile) * kScanTile;
   const int64_t e0 = base + static_cast<int64_t>(t) * kScanItems;
+  // uf and the core flags come from earlier kernels: plain loads, both
+  // issued together (a root is a core cell that is its own parent)
   uint32_t f[kScanItems];
+  int32_t u[kScanItems];
   uint32_t cnt = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    f[j] = (e0 + j < C && ld_acquire(uf + e0 + j) == static_cast<int32_t>(e0 + j))
-               ? flag_rank[e0 + j]
-               : 0u;
+    const int64_t i = e0 + j < C ? e0 + j : C - 1;
+    u[j] = uf[i];
+    f[j] = flag_rank[i];
+  }
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    f[j] = (e0 + j < C && u[j] == static_cast<int32_t>(e0 + j)) ? f[j] : 0u;
     cnt += f[j];
   }
   uint32_t ttot;
