@@ -2496,7 +2496,13 @@ __global__ __launch_bounds__(256) void k_label_direct(
     const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
     const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
     int32_t* __restrict__ labels, int64_t n) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  // Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous eighth of
+  // the sorted points.  In a batch a cloud's points stay inside its own input
+  // range, so one cloud's scattered label writes then gather in one XCD's L2
+  // (whole lines leave it) instead of partial lines from all eight.
+  const uint32_t g = gridDim.x, x = blockIdx.x & 7u, q8 = g >> 3, r8 = g & 7u;
+  const uint32_t blk = x * q8 + min(x, r8) + (blockIdx.x >> 3);
+  const int64_t i = static_cast<int64_t>(blk) * 256 + threadIdx.x;
   if (i >= n) return;
   const bool in_a = (prm->npasses & 1) != 0;
   const uint32_t dst = (in_a ? idx_a : idx_b)[i];
