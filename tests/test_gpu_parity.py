@@ -149,6 +149,21 @@ def test_dense_cell_long_runs(dt, clusters):
     assert np.array_equal(labels, oracle.dbscan2d(X, 0.5, 1))
 
 
+def test_twelve_million_points_wide_buckets(dt):
+    """Between 10.5M and 2^24 points the label buckets are wider than 20,480
+    positions and the unpermute runs its 1,024-thread, 128 KB workgroups;
+    a cell of 1.5M points gives some buckets long runs (the chunked path of
+    that instantiation).  blobs10m, the dense cell and 0.5M shifted points,
+    shuffled, against the oracle."""
+    X, eps, ms = W.blobs10m()
+    rng = np.random.default_rng(77)
+    dense = rng.uniform(3000.0, 3000.2, (1_500_000, 2)).astype(np.float32)
+    X2 = np.concatenate([X, dense, X[:500_000] + np.float32(2000.0)])
+    X2 = X2[rng.permutation(len(X2))]
+    labels = gpu_labels(dt, X2, eps, ms)
+    assert np.array_equal(labels, oracle.dbscan2d(X2, eps, ms))
+
+
 def test_back_to_back_calls_reuse_the_workspace(dt):
     """Calls that reuse a workspace must not see the previous call's look-back
     words (a stale inclusive prefix once gave the last scan tile prefix 0):
