@@ -2446,7 +2446,7 @@ __device__ __forceinline__ void label_tile(
       else
         s_buf[s_cnt[b] + rk[k]] =
             (static_cast<uint64_t>(static_cast<uint32_t>(lab[k])) << 32) | dst[k];
-    } else if (k * T + t < tile_n) {
+    } else if (static_cast<uint32_t>(k * T + t) < tile_n) {
       atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
     }
   }
