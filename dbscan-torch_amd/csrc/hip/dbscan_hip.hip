@@ -967,8 +967,8 @@ __device__ void grid_barrier(uint32_t* counter, uint32_t target, int32_t* error)
 }
 
 // The radix passes from kTailPass on, all in one launch of one workgroup per
-// CU (they are needed only for keys wider than 32 bits; for the usual key the
-// launch returns at once): each pass walks the tiles, and a grid barrier
+// CU (they are needed only for keys wider than four digits; for the usual key
+// the launch returns at once): each pass walks the tiles, and a grid barrier
 // separates the passes.  Pass p reads buffer set a if p is odd, else b.
 template <bool kSeg, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
@@ -3107,9 +3107,9 @@ bool config_ok(const dbscan_hip_config& c) {
 }
 
 // Passes from kTailPass on are needed only for keys wider than 4 digits of
-// the configuration's radix_bits (36 bits at 9); they run in one launch (k_sort_tail) of one workgroup
-// per CU that walks the tiles, so that for the usual narrow key they cost a
-// single launch that returns at once.  One workgroup per CU is always
+// the configuration's radix_bits (36 bits at 9); they run in one launch
+// (k_sort_tail) of one workgroup per CU that walks the tiles, so that for the
+// usual narrow key they cost a single launch that returns at once.  One workgroup per CU is always
 // co-resident, which the walk and the barrier between passes need: a
 // workgroup waits only on tiles of lower index, held by workgroups that are
 // resident or done.  (The walk is a separate instantiation: its loop would
