@@ -1244,6 +1244,10 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
     const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
     const uint32_t yhi = yr <= 0xFFFFFFFDu ? yr + 2u : 0xFFFFFFFFu;
     int32_t* row = ngh + c64 * 10;
+    // Every key compared below lies in a known column (inside that column's
+    // [lo, hi)), so only its y' word (the low half) is loaded.
+    const uint32_t* __restrict__ key_y = reinterpret_cast<const uint32_t*>(cell_key);
+    static_assert(sizeof(uint64_t) == 2 * sizeof(uint32_t), "");
     // The five columns' searches run in lock-step, so that each step is one
     // round of five independent loads rather than five dependent ones.
     int32_t lo[5], hi[5], base[5], len[5];
@@ -1260,15 +1264,14 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
     // (all five loads of a step issued before any is used: a finished or
     // empty search loads cell c, which exists)
     for (uint32_t it = 0; it < 40; ++it) {  // len < 2^31: at most 32 halvings
-      uint64_t v[5];
+      uint32_t v[5];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) v[k] = cell_key[len[k] > 0 ? base[k] + (len[k] >> 1) : c];
+      for (int k = 0; k < 5; ++k) v[k] = key_y[2 * (len[k] > 0 ? base[k] + (len[k] >> 1) : c)];
       bool more = false;
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        const uint64_t colbase = static_cast<uint64_t>(static_cast<int64_t>(xr) + (k - 2)) << 32;
         const int32_t half = len[k] >> 1;
-        const bool right = v[k] < (colbase | ylo);
+        const bool right = v[k] < ylo;
         if (len[k] > 0) {
           base[k] = right ? base[k] + half + 1 : base[k];
           len[k] = right ? len[k] - half - 1 : half;
@@ -1279,18 +1282,16 @@ __global__ __launch_bounds__(256) void k_ngh(Params* __restrict__ prm,
     }
     // the column's cells up to y' <= yhi: at most 5 (one per y'), keys
     // ascending, so the qualifying ones are a prefix
-    uint64_t w[5][5];
+    uint32_t w[5][5];
 #pragma unroll
     for (int k = 0; k < 5; ++k)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) w[k][q] = cell_key[base[k] + q < hi[k] ? base[k] + q : c];
+      for (int q = 0; q < 5; ++q) w[k][q] = key_y[2 * (base[k] + q < hi[k] ? base[k] + q : c)];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      const uint64_t lim =
-          (static_cast<uint64_t>(static_cast<int64_t>(xr) + (k - 2)) << 32) | yhi;
       int32_t cnt = 0;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) cnt += (base[k] + q < hi[k] && w[k][q] <= lim) ? 1 : 0;
+      for (int q = 0; q < 5; ++q) cnt += (base[k] + q < hi[k] && w[k][q] <= yhi) ? 1 : 0;
       row[2 * k] = base[k];
       row[2 * k + 1] = base[k] + cnt;
     }
