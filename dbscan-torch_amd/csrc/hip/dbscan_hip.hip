@@ -2379,8 +2379,6 @@ struct LabelForm {
   using Slot = std::conditional_t<kPacked, uint32_t, uint64_t>;  // 64 KB of LDS either way
 };
 static_assert(kLabelThreads >= kMaxBuckets, "a thread per bucket");
-static_assert(LabelForm<true>::kTile <= 1 << 16 && LabelForm<false>::kTile <= 1 << 16,
-              "a run's bounds within its tile");
 // A run of more pairs than this marks its bucket for the chunked unpermute.
 constexpr uint32_t kLongRun = 256;
 
@@ -2390,7 +2388,9 @@ constexpr uint32_t kLongRun = 256;
 // them, in that order, to its own slice of the pair arrays (the tile's
 // sorted positions), one contiguous write; a table gives where each bucket's
 // run starts in the slice: row b, column tile, row nb holding the tile's
-// size.  No tile waits for another; the order inside a run is free.
+// size (the table lives in the radix passes' look-back words, free by now).
+// A bucket that gets a run of more than kLongRun pairs is flagged in
+// long_runs.  No tile waits for another; the order inside a run is free.
 template <bool kPacked>
 __device__ __forceinline__ void label_tile(
     Params* __restrict__ prm, const uint32_t* __restrict__ idx_a,
