@@ -3098,7 +3098,8 @@ dbscan_hip_config default_config(int64_t n) {
   if (n <= 31622) return dbscan_hip_config{8, 8, 2048, 16384};
   if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 4096};
   if (n <= 3162277) return dbscan_hip_config{8, 7, 8192, 8192};
-  return dbscan_hip_config{12, 9, 4096, 8192};
+  if (n <= (int64_t{1} << 24)) return dbscan_hip_config{12, 9, 4096, 8192};
+  return dbscan_hip_config{12, 9, 8192, 8192};  // mostly batches of many clouds
 }
 
 bool config_ok(const dbscan_hip_config& c) {
