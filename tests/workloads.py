@@ -23,7 +23,9 @@ Named workloads:
          eps_i = f32(U[0.2, 1.0]), min_samples_i from {5, 10, 20}
   serve    open-loop Poisson stream per rate (seed [5, rate]): N_i from the
          mixture 70% log-U[1k,50k], 25% log-U[50k,1M], 5% log-U[1M,10M],
-         eps_i and min_samples_i as clouds2048, clouds at blobs10k's density
+         eps_i and min_samples_i as clouds2048, clouds at blobs10k's density;
+         request i of the stream at `rate` has its own cloud,
+         cloud_at_density(N_i, 100000 * rate + i) (serve_request_cloud)
 """
 
 from __future__ import annotations
@@ -120,6 +122,17 @@ def serve_stream(rate: float, count: int, seed: int = 5):
     eps = rng.uniform(0.2, 1.0, count).astype(np.float32)
     ms = rng.choice(np.array([5, 10, 20]), count)
     return arrivals, ns, eps, ms
+
+
+# The pinned served sets: the first SERVE_PINNED_COUNT requests of the streams
+# at these rates, each with its own cloud (tests/golden/streams.json).
+SERVE_PINNED_RATES = (1000, 5000)
+SERVE_PINNED_COUNT = 300
+
+
+def serve_request_cloud(rate: int, i: int, n: int) -> np.ndarray:
+    """The points of request i of the stream at `rate` req/s (n = its N_i)."""
+    return cloud_at_density(int(n), 100_000 * int(rate) + int(i))
 
 
 WORKLOADS = {"blobs10k": blobs10k, "lattice1m": lattice1m, "blobs10m": blobs10m, "skewed10m": skewed10m}
