@@ -8,6 +8,12 @@ constant-density workload (1,000 Gaussian blobs, std 0.5, centres in
 min_samples 10, seed 2).  Prints one JSON line:
 
   value          Mpts/s over K timed plain dbscan2d calls (input already in HBM)
+  call_ms        median, p10 and p90 of the per-call device time (HIP events
+                 on the launch stream between the timed calls)
+  labels_ok      the timed calls' labels against the reference's digest
+                 (tests/golden/golden.json); a mismatch exits non-zero
+  configs        the other single-call workloads, lattice1m (1.07M points)
+                 and skewed10m (10M, skewed), measured the same way
   roofline       the dominant kernel's algorithmic bytes per launch over its
                  mean launch time, from HIP events recorded on the launch
                  stream before every kernel over K further calls, against the
@@ -116,7 +122,83 @@ def measured_traffic(group, npasses):
     return sum(vals) / len(vals) if vals else None
 
 
-def cpu_baseline(X, eps, ms, reps=3):
+GOLDEN = os.path.join(ROOT, "tests", "golden", "golden.json")
+
+
+def labels_digest(labels) -> str:
+    import hashlib
+
+    import numpy as np
+
+    return hashlib.sha256(np.ascontiguousarray(labels.cpu().numpy(), dtype=np.int32)
+                          .tobytes()).hexdigest()
+
+
+def golden_digest(workload: str):
+    if not os.path.exists(GOLDEN):
+        return None
+    with open(GOLDEN) as f:
+        return json.load(f).get(workload, {}).get("sha256")
+
+
+def time_calls(dbscan_torch, Xd, eps, ms, steps, warmup):
+    """`warmup` untimed calls, then `steps` back-to-back calls bracketed by
+    synchronisations: wall ms per call, per-call device ms (events recorded on
+    the launch stream between the calls), the last labels."""
+    import torch
+
+    for _ in range(warmup):
+        dbscan_torch.dbscan2d(Xd, eps, ms)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    for k in range(steps):
+        labels = dbscan_torch.dbscan2d(Xd, eps, ms)
+        ev[k + 1].record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    per = sorted(ev[k].elapsed_time(ev[k + 1]) for k in range(steps))
+    dbscan_torch.check_device_errors()
+    return wall, per, labels
+
+
+def pct(sorted_vals, p):
+    return sorted_vals[min(len(sorted_vals) - 1, int(round(p / 100.0 * (len(sorted_vals) - 1))))]
+
+
+def config_line(dbscan_torch, name, steps, warmup):
+    """One single-call workload measured like the headline one."""
+    import torch
+
+    from tests import workloads as W
+
+    X, eps, ms = W.WORKLOADS[name]()
+    n = len(X)
+    Xd = torch.from_numpy(X).cuda()
+    wall, per, labels = time_calls(dbscan_torch, Xd, eps, ms, steps, warmup)
+    digest = labels_digest(labels)
+    want = golden_digest(name)
+    cells = None
+    if os.path.exists(GOLDEN):
+        with open(GOLDEN) as f:
+            cells = json.load(f).get(name, {}).get("n_cells")
+    out = {"n_points": n, "eps": eps, "min_samples": ms, "steps": steps,
+           "ms_per_call": round(wall, 4),
+           "call_ms": {"median": round(pct(per, 50), 4), "p10": round(pct(per, 10), 4),
+                       "p90": round(pct(per, 90), 4)},
+           "mpts_per_s": round(n / (wall * 1e-3) / 1e6, 1),
+           "labels_ok": (digest == want) if want else None}
+    if cells:
+        b = 107 * n + 40 * cells
+        out["e2e_roofline"] = {"bytes_model": "107*N + 40*C", "bytes": b,
+                               "frac": round(b / (wall * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    del Xd, labels
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(X, eps, ms, reps=5):
     import numpy as np
     import torch
 
@@ -280,6 +362,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="blobs10m", choices=["lattice1m", "blobs10m", "skewed10m"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the lattice1m / skewed10m single-call lines")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 2,048-cloud batched-vs-looped leg")
     ap.add_argument("--batched-only", action="store_true",
@@ -330,17 +414,10 @@ def main() -> int:
     X, eps, ms = W.WORKLOADS[args.workload]()
     n = len(X)
     Xd = torch.from_numpy(X).cuda()
-    for _ in range(args.warmup):
-        dbscan_torch.dbscan2d(Xd, eps, ms)
-    torch.cuda.synchronize()
-
     # timed region: K plain calls, as a user makes them
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        labels = dbscan_torch.dbscan2d(Xd, eps, ms)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
+    ms_per_step, per_call, labels = time_calls(dbscan_torch, Xd, eps, ms, args.steps, args.warmup)
+    want = golden_digest(args.workload)
+    labels_ok = (labels_digest(labels) == want) if want else None
     # per-kernel durations from a second run with HIP events before every
     # kernel on the launch stream (the events add ~5 us per launch, so this run
     # is kept out of `value`)
@@ -394,19 +471,35 @@ def main() -> int:
                          "frac": round(e2e_gbs / HBM_PEAK_GBS, 4)},
         "kernels": {k: {"launches": v["launches"], "ms_per_launch": round(v["ms_per_launch"], 4),
                         "gbs": round(v["gbs"], 1)} for k, v in per_kernel.items()},
+        "call_ms": {"median": round(pct(per_call, 50), 4), "p10": round(pct(per_call, 10), 4),
+                    "p90": round(pct(per_call, 90), 4)},
+        "labels_ok": labels_ok,
         "event_timed_ms_per_step": round(event_total_ms / args.steps, 4),
         "error_bits": err,
     }
+    if not args.no_configs:
+        del Xd, labels
+        torch.cuda.empty_cache()
+        out["configs"] = {}
+        for name in ("lattice1m", "skewed10m"):
+            if name == args.workload:
+                continue
+            out["configs"][name] = config_line(dbscan_torch, name, args.steps, args.warmup)
+            print(f"[bench] {name}: {out['configs'][name]}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, eps, ms)
     if not args.no_batched:
-        del Xd, labels
         torch.cuda.empty_cache()
         out["batched"] = batched_leg(dbscan_torch, 3, 1)
         torch.cuda.empty_cache()
     if not args.no_serving:
         out["serving"] = serving_leg(dbscan_torch, rates, args.serve_requests)
     print(json.dumps(out), flush=True)
+    bad = [k for k, v in [(args.workload, labels_ok)] +
+           [(k, v["labels_ok"]) for k, v in out.get("configs", {}).items()] if v is False]
+    if bad:
+        print(f"bench.py: labels differ from the reference digest for {bad}", file=sys.stderr)
+        return 3
     return 0
 
 

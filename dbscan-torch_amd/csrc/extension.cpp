@@ -18,7 +18,9 @@
 #include <c10/hip/HIPStream.h>
 
 #include <algorithm>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <optional>
 #include <string>
 #include <tuple>
@@ -45,6 +47,133 @@ void check_status(int rc) {
   TORCH_CHECK(rc == DBSCAN_HIP_OK, "dbscan2d: ", dbscan2d_hip_status_string(rc));
 }
 
+// Text of a non-zero device error word: each set bit named.
+std::string device_error_text(int32_t bits) {
+  std::string why;
+  for (int b = 0; b < 31; ++b) {
+    const int32_t bit = static_cast<int32_t>(1u << b);
+    if (!(bits & bit)) continue;
+    const char* name = dbscan2d_hip_device_error_name(bit);
+    if (!why.empty()) why += "; ";
+    why += name ? name : ("unknown bit " + std::to_string(b));
+  }
+  char hex[16];
+  std::snprintf(hex, sizeof(hex), "0x%x", static_cast<unsigned>(bits));
+  return std::string("device-side check failed (error bits ") + hex + ": " + why +
+         "); every label of that call is INT32_MIN";
+}
+
+void raise_for_device_error(int32_t bits, const char* which = "dbscan2d: ") {
+  TORCH_CHECK(bits == 0, which, device_error_text(bits));
+}
+
+// Error words of calls that returned device labels without a host
+// synchronisation: each such call has its last kernel write its word into a
+// pinned slot (dbscan2d_hip_checked) and records an event behind it; the
+// words of finished calls are examined, oldest first, at the start of every
+// later call on the device and by check_device_errors(), which raise for a
+// non-zero word.  (A call whose labels come back to the host checks its own
+// word before returning.)
+struct ErrorWatch {
+  static constexpr int kSlots = 64;
+  int32_t* words = nullptr;  // pinned host memory, one word per slot
+  hipEvent_t ev[kSlots] = {};
+  std::deque<int> pending;   // slots in enqueue order
+  int next = 0;
+};
+std::mutex g_watch_mu;
+std::map<int, ErrorWatch> g_watch;
+
+ErrorWatch& watch_for(int device) {
+  ErrorWatch& w = g_watch[device];
+  if (!w.words) {
+    void* p = nullptr;
+    C10_HIP_CHECK(hipHostMalloc(&p, sizeof(int32_t) * ErrorWatch::kSlots, hipHostMallocDefault));
+    w.words = static_cast<int32_t*>(p);
+    for (auto& e : w.ev) C10_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  return w;
+}
+
+// True while `stream` is being captured into a HIP graph: event queries and
+// waits are not allowed then, so captured calls are neither polled nor
+// watched (a replayed call that fails a check still leaves its labels
+// INT32_MIN).
+bool capturing(hipStream_t stream) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// Examines the pending words of `device` (waiting for them if `wait`);
+// raises for the first non-zero one.
+void poll_device_errors(int device, bool wait) {
+  int32_t bits = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_watch_mu);
+    auto it = g_watch.find(device);
+    if (it == g_watch.end()) return;
+    ErrorWatch& w = it->second;
+    while (!w.pending.empty() && bits == 0) {
+      const int slot = w.pending.front();
+      if (wait) {
+        C10_HIP_CHECK(hipEventSynchronize(w.ev[slot]));
+      } else {
+        const hipError_t q = hipEventQuery(w.ev[slot]);
+        if (q == hipErrorNotReady) break;
+        C10_HIP_CHECK(q);
+      }
+      w.pending.pop_front();
+      bits = w.words[slot];
+    }
+  }
+  raise_for_device_error(bits, "dbscan2d: an earlier call's ");
+}
+
+// A pinned slot for the error word of a call about to be enqueued on
+// `device` (the oldest slot is waited for and examined if all are pending).
+struct WatchSlot {
+  int device = -1, slot = -1;
+  int32_t* word = nullptr;
+};
+
+WatchSlot watch_begin(int device) {
+  int32_t bits = 0;
+  WatchSlot ws;
+  {
+    std::lock_guard<std::mutex> lock(g_watch_mu);
+    ErrorWatch& w = watch_for(device);
+    if (static_cast<int>(w.pending.size()) == ErrorWatch::kSlots) {
+      const int old = w.pending.front();
+      C10_HIP_CHECK(hipEventSynchronize(w.ev[old]));
+      w.pending.pop_front();
+      bits = w.words[old];
+    }
+    ws.device = device;
+    ws.slot = w.next;
+    w.next = (w.next + 1) % ErrorWatch::kSlots;
+    ws.word = w.words + ws.slot;
+  }
+  raise_for_device_error(bits, "dbscan2d: an earlier call's ");
+  return ws;
+}
+
+// Marks the slot's word as due once `stream` reaches this point.
+void watch_end(const WatchSlot& ws, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(g_watch_mu);
+  ErrorWatch& w = g_watch[ws.device];
+  C10_HIP_CHECK(hipEventRecord(w.ev[ws.slot], stream));
+  w.pending.push_back(ws.slot);
+}
+
+// Waits for `stream` and raises if the call that last used `workspace` failed
+// a device-side check.
+void check_call(const void* workspace, hipStream_t stream) {
+  int32_t bits = 0;
+  const int rc = dbscan2d_hip_check(workspace, stream, &bits);
+  if (rc != DBSCAN_HIP_ERR_DEVICE) check_status(rc);
+  raise_for_device_error(bits);
+}
+
 // Launch configurations by input size, installed by the autotuner
 // (dbscan_torch.autotune); empty: the library's built-in table.
 std::vector<std::pair<int64_t, dbscan_hip_config>> g_tuning;
@@ -68,9 +197,18 @@ struct DeviceRun {
   torch::Tensor workspace;  // uint8 workspace (kept alive by the debug binding)
 };
 
+// How a call's device-side error word is checked: before returning (the
+// result goes back to the host, or the call waits anyway), by the watch
+// (device results), or not at all (the debug variant returns the word).
+enum class Check { kNow, kWatch, kNone };
+
+Check check_for(const torch::Tensor& X, bool waits) {
+  return (!X.is_cuda() || waits) ? Check::kNow : Check::kWatch;
+}
+
 // Runs the pipeline; stage_ms may be null.  Returns device tensors.
 DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, float* stage_ms,
-                     const dbscan_hip_config* cfg = nullptr) {
+                     const dbscan_hip_config* cfg = nullptr, Check check = Check::kWatch) {
   c10::Device device = X.device();
   if (!X.is_cuda()) {
     TORCH_CHECK(c10::hip::device_count() > 0,
@@ -79,6 +217,9 @@ DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, fl
     device = c10::Device(c10::kCUDA, c10::hip::current_device());
   }
   c10::DeviceGuard guard(device);
+  hipStream_t stream = c10::hip::getCurrentHIPStream(device.index()).stream();
+  const bool in_graph = capturing(stream);
+  if (!in_graph) poll_device_errors(device.index(), false);
   DeviceRun r;
   r.Xd = X.to(device).contiguous();
   const int64_t n = r.Xd.size(0);
@@ -88,15 +229,19 @@ DeviceRun run_device(const torch::Tensor& X, double eps, int64_t min_samples, fl
   auto opts = torch::TensorOptions().device(device);
   r.workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
   r.labels = torch::empty({n}, opts.dtype(torch::kInt32));
-  hipStream_t stream = c10::hip::getCurrentHIPStream(device.index()).stream();
+  const bool watch = check == Check::kWatch && !in_graph && !stage_ms;
+  WatchSlot slot;
+  if (watch) slot = watch_begin(device.index());
   const int rc =
       stage_ms ? dbscan2d_hip_profile(r.Xd.data_ptr<float>(), n, eps, min_samples,
                                       r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(),
                                       ws_bytes, stream, stage_ms)
-               : dbscan2d_hip_configured(r.Xd.data_ptr<float>(), n, eps, min_samples,
-                                         r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(),
-                                         ws_bytes, stream, cfg ? cfg : config_for(n));
+               : dbscan2d_hip_checked(r.Xd.data_ptr<float>(), n, eps, min_samples,
+                                      r.labels.data_ptr<int32_t>(), r.workspace.data_ptr(),
+                                      ws_bytes, stream, cfg ? cfg : config_for(n), slot.word);
   check_status(rc);
+  if (watch) watch_end(slot, stream);
+  else if (check == Check::kNow && !in_graph) check_call(r.workspace.data_ptr(), stream);
   return r;
 }
 
@@ -112,7 +257,7 @@ torch::Tensor dbscan2d(torch::Tensor X, double eps, int64_t min_samples) {
   if (X.size(0) == 0) {
     return torch::empty({0}, X.options().dtype(torch::kInt32));
   }
-  return finish(X, run_device(X, eps, min_samples, nullptr).labels);
+  return finish(X, run_device(X, eps, min_samples, nullptr, nullptr, check_for(X, false)).labels);
 }
 
 // dbscan2d with an explicit launch configuration (no reference counterpart;
@@ -122,7 +267,7 @@ torch::Tensor dbscan2d_configured(torch::Tensor X, double eps, int64_t min_sampl
   check_input(X, eps, min_samples);
   if (X.size(0) == 0) return torch::empty({0}, X.options().dtype(torch::kInt32));
   const dbscan_hip_config cfg = to_config(config);
-  return finish(X, run_device(X, eps, min_samples, nullptr, &cfg).labels);
+  return finish(X, run_device(X, eps, min_samples, nullptr, &cfg, check_for(X, false)).labels);
 }
 
 void set_tuning_table(std::vector<std::vector<int64_t>> rows) {
@@ -160,7 +305,7 @@ std::pair<torch::Tensor, std::map<std::string, float>> dbscan2d_profile(torch::T
   if (X.size(0) == 0) {
     labels = torch::empty({0}, X.options().dtype(torch::kInt32));
   } else {
-    labels = finish(X, run_device(X, eps, min_samples, ms).labels);
+    labels = finish(X, run_device(X, eps, min_samples, ms, nullptr, Check::kNow).labels);
   }
   std::map<std::string, float> out;
   for (int i = 0; i < DBSCAN_HIP_NUM_STAGES; ++i) out[dbscan2d_hip_stage_name(i)] = ms[i];
@@ -183,7 +328,7 @@ std::pair<torch::Tensor, std::map<std::string, float>> dbscan2d_profile(torch::T
 std::vector<torch::Tensor> dbscan2d_hip_debug(torch::Tensor X, double eps, int64_t min_samples) {
   check_input(X, eps, min_samples);
   TORCH_CHECK(X.size(0) > 0, "debug variant requires n > 0");
-  DeviceRun r = run_device(X, eps, min_samples, nullptr);
+  DeviceRun r = run_device(X, eps, min_samples, nullptr, nullptr, Check::kNone);
   const int64_t n = r.Xd.size(0);
   dbscan_hip_layout L;
   check_status(dbscan2d_hip_layout(n, &L));
@@ -211,6 +356,99 @@ std::vector<torch::Tensor> dbscan2d_hip_debug(torch::Tensor X, double eps, int64
   out.push_back(slice(L.uf, C, 4, torch::kInt32).clone());
   out.push_back(slice(L.cell_cluster, C, 4, torch::kInt32).clone());
   return out;
+}
+
+// The reference's debug entry point (csrc/cuda/dbscan_cuda.cu:772-800,
+// bound at csrc/extension.cpp:76-79): labels and ten stage intermediates, in
+// the reference's order and in the reference CPU path's point and cell order
+// (cells by the signed int64 key, points of a cell in input order; the
+// library's internal cell order differs inside a column, and each cell's
+// reference index maps between the two):
+//   0 labels (n,) int32                    1 keys_sorted (n,) int64, the signed
+//                                            key u32(kx) << 32 | u32(ky) per point
+//   2 perm (n,) int32 (sorted -> input)    3 xs (n,) float32   4 ys (n,) float32
+//   5 is_core (n,) uint8                   6 cell_start (C+1,) int32
+//   7 cluster_id_per_cell (C,) int32       8 uf (C,) int32, each cell's root
+//   9 sorted_labels_pre_border (n,) int32, core labels and -1 elsewhere
+//  10 sorted_labels_final (n,) int32, after border assignment
+// Device tensors; no error is raised for a failed device-side check (the
+// labels are then INT32_MIN).
+std::vector<torch::Tensor> dbscan2d_cuda_debug(torch::Tensor X, double eps, int64_t min_samples) {
+  check_input(X, eps, min_samples);
+  TORCH_CHECK(X.size(0) > 0, "debug variant requires n > 0");
+  TORCH_CHECK(static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(min_samples))) >= 1,
+              "debug variant: min_samples narrows below 1 (every label is -1, no stage runs)");
+  DeviceRun r = run_device(X, eps, min_samples, nullptr, nullptr, Check::kNone);
+  const int64_t n = r.Xd.size(0);
+  dbscan_hip_layout L;
+  check_status(dbscan2d_hip_layout(n, &L));
+  auto ws = r.workspace;
+  auto slice = [&](size_t off, int64_t count, int64_t elem, torch::ScalarType t) {
+    return ws.narrow(0, static_cast<int64_t>(off), count * elem).view(t);
+  };
+  const auto info = slice(L.info, 8, 4, torch::kInt32).cpu();
+  const int64_t kxmin = info[0].item<int64_t>(), kymin = info[2].item<int64_t>();
+  const int64_t C = info[4].item<int64_t>(), npasses = info[5].item<int64_t>();
+  const bool in_a = (npasses & 1) != 0;
+  const auto i64 = torch::kInt64;
+  auto perm = slice(in_a ? L.idx_a : L.idx_b, n, 4, torch::kInt32);
+  auto xs = slice(in_a ? L.xs_a : L.xs_b, n, 4, torch::kFloat32);
+  auto ys = slice(in_a ? L.ys_a : L.ys_b, n, 4, torch::kFloat32);
+  auto lab_sorted = slice(in_a ? L.idx_b : L.idx_a, n, 4, torch::kInt32);  // border labels
+  auto point_cell = slice(L.point_cell, n, 4, torch::kInt32).to(i64);
+  auto is_core = slice(L.is_core, n, 1, torch::kUInt8);
+  auto cell_start = slice(L.cell_start, C + 1, 4, torch::kInt32).to(i64);
+  auto cell_key = slice(L.cell_key, C, 8, torch::kInt64);
+  auto cell_ref = slice(L.cell_ref, C, 4, torch::kInt32).to(i64);
+  auto uf = slice(L.uf, C, 4, torch::kInt32).clone();
+  auto cell_cluster = slice(L.cell_cluster, C, 4, torch::kInt32);
+  // reference cell order: sizes by reference index, their running sum
+  auto sizes = cell_start.narrow(0, 1, C) - cell_start.narrow(0, 0, C);
+  auto sizes_ref = torch::empty_like(sizes).index_put_({cell_ref}, sizes);
+  auto ref_start = torch::cat({torch::zeros({1}, sizes.options()), sizes_ref.cumsum(0)});
+  auto pos = torch::arange(n, point_cell.options());
+  auto new_pos = ref_start.index({cell_ref.index({point_cell})}) + pos -
+                 cell_start.index({point_cell});
+  auto remap = [&](const torch::Tensor& a) {
+    return torch::empty_like(a).index_put_({new_pos}, a);
+  };
+  // the signed key of each cell: kx * 2^32 + u32(ky)
+  const int64_t m32 = 0xFFFFFFFFll;
+  auto low32 = [&](const torch::Tensor& t) { return torch::bitwise_and(t, m32); };
+  auto kx_u = low32(low32(torch::bitwise_right_shift(cell_key, 32)) + (kxmin & m32));
+  auto ky_u = low32(low32(cell_key) + (kymin & m32));
+  auto kx_s = kx_u - (kx_u >= (int64_t{1} << 31)).to(i64) * (int64_t{1} << 32);
+  auto key = kx_s * (int64_t{1} << 32) + ky_u;
+  auto core = is_core.to(torch::kBool);
+  auto core_lab = cell_cluster.index({point_cell});
+  auto minus1 = torch::full_like(core_lab, -1);
+  auto pre = torch::where(core, core_lab, minus1);
+  auto fin = torch::where(core, core_lab, lab_sorted);
+  auto cc_ref = torch::empty_like(cell_cluster).index_put_({cell_ref}, cell_cluster);
+  return {r.labels,
+          remap(key.index({point_cell})),
+          remap(perm),
+          remap(xs),
+          remap(ys),
+          remap(is_core),
+          ref_start.to(torch::kInt32),
+          cc_ref,
+          uf,
+          remap(pre),
+          remap(fin)};
+}
+
+void check_device_errors(std::optional<int64_t> device) {
+  if (device) {
+    poll_device_errors(static_cast<int>(*device), true);
+    return;
+  }
+  std::vector<int> devs;
+  {
+    std::lock_guard<std::mutex> lock(g_watch_mu);
+    for (const auto& kv : g_watch) devs.push_back(kv.first);
+  }
+  for (int d : devs) poll_device_errors(d, true);
 }
 
 namespace {
@@ -259,18 +497,27 @@ torch::Tensor dbscan2d_batched(torch::Tensor X, std::vector<int64_t> offsets,
     device = c10::Device(c10::kCUDA, c10::hip::current_device());
   }
   c10::DeviceGuard guard(device);
+  hipStream_t stream = c10::hip::getCurrentHIPStream(device.index()).stream();
+  const bool in_graph = capturing(stream);
+  if (!in_graph) poll_device_errors(device.index(), false);
   auto Xd = X.to(device).contiguous();
   size_t ws_bytes = 0;
   check_status(dbscan2d_hip_batched_workspace_bytes(n, B, &ws_bytes));
   auto opts = torch::TensorOptions().device(device);
   auto workspace = torch::empty({static_cast<int64_t>(ws_bytes)}, opts.dtype(torch::kUInt8));
   auto labels = torch::empty({n}, opts.dtype(torch::kInt32));
-  hipStream_t stream = c10::hip::getCurrentHIPStream(device.index()).stream();
   int bad = -1;
-  check_status(dbscan2d_hip_batched(Xd.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
-                                    b.ms.data(), B, labels.data_ptr<int32_t>(),
-                                    workspace.data_ptr(), ws_bytes, stream, &bad,
-                                    config_for(n)));
+  const bool watch = X.is_cuda() && !in_graph;
+  WatchSlot slot;
+  if (watch) slot = watch_begin(device.index());
+  check_status(dbscan2d_hip_batched_checked(Xd.data_ptr<float>(), n, b.offsets.data(),
+                                            b.eps.data(), b.ms.data(), B,
+                                            labels.data_ptr<int32_t>(), workspace.data_ptr(),
+                                            ws_bytes, stream, &bad, config_for(n), slot.word));
+  if (watch)
+    watch_end(slot, stream);
+  else if (!in_graph)
+    check_call(workspace.data_ptr(), stream);
   return finish(X, labels);
 }
 
@@ -280,7 +527,7 @@ torch::Tensor dbscan2d_batched(torch::Tensor X, std::vector<int64_t> offsets,
 // current stream, without allocating.  Returns nothing; the work is enqueued.
 void dbscan2d_batched_out(torch::Tensor X, std::vector<int64_t> offsets, std::vector<double> eps,
                           std::vector<int64_t> min_samples, torch::Tensor labels,
-                          torch::Tensor workspace) {
+                          torch::Tensor workspace, std::optional<torch::Tensor> error_word) {
   const Batch b = check_batch(X, offsets, eps, min_samples);
   const int64_t n = X.size(0), B = static_cast<int64_t>(eps.size());
   TORCH_CHECK(X.is_cuda() && labels.is_cuda() && workspace.is_cuda() &&
@@ -292,14 +539,24 @@ void dbscan2d_batched_out(torch::Tensor X, std::vector<int64_t> offsets, std::ve
               "dbscan2d_batched_out: labels must be int32 with at least N elements");
   TORCH_CHECK(workspace.scalar_type() == torch::kUInt8,
               "dbscan2d_batched_out: workspace must be uint8");
+  int32_t* err = nullptr;
+  if (error_word) {
+    const torch::Tensor& e = *error_word;
+    TORCH_CHECK(e.scalar_type() == torch::kInt32 && e.numel() >= 1 && e.is_contiguous() &&
+                    (e.is_cuda() || e.is_pinned()),
+                "dbscan2d_batched_out: error_word must be an int32 tensor in device or pinned "
+                "host memory");
+    err = e.data_ptr<int32_t>();
+    if (n == 0) *err = 0;  // nothing is enqueued (a device word is left as it is)
+  }
   if (n == 0) return;
   c10::DeviceGuard guard(X.device());
   hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
   int bad = -1;
-  check_status(dbscan2d_hip_batched(X.data_ptr<float>(), n, b.offsets.data(), b.eps.data(),
-                                    b.ms.data(), B, labels.data_ptr<int32_t>(),
-                                    workspace.data_ptr(), static_cast<size_t>(workspace.numel()),
-                                    stream, &bad, config_for(n)));
+  check_status(dbscan2d_hip_batched_checked(
+      X.data_ptr<float>(), n, b.offsets.data(), b.eps.data(), b.ms.data(), B,
+      labels.data_ptr<int32_t>(), workspace.data_ptr(), static_cast<size_t>(workspace.numel()),
+      stream, &bad, config_for(n), err));
 }
 
 int64_t dbscan2d_batched_workspace_bytes(int64_t total_points, int64_t num_clouds) {
@@ -465,6 +722,20 @@ PYBIND11_MODULE(_C, m) {
         pybind11::arg("min_samples"),
         "Debug variant: returns [labels, info, perm, xs, ys, point_cell, is_core, cell_start, "
         "cell_key, cell_flags, cell_ref, uf, cell_cluster]. Not part of the public API.");
+  m.def("dbscan2d_cuda_debug", &dbscan2d_cuda_debug, pybind11::arg("X"), pybind11::arg("eps"),
+        pybind11::arg("min_samples"),
+        "Debug variant: returns a list of per-stage intermediate tensors in the reference's "
+        "order (labels, keys_sorted, perm, xs, ys, is_core, cell_start, cluster_id_per_cell, "
+        "uf, sorted_labels_pre_border, sorted_labels_final). Not part of the public API.");
+  m.def("check_device_errors", &check_device_errors, pybind11::arg("device") = pybind11::none(),
+        "Waits for the calls whose device labels were returned without a host "
+        "synchronisation and raises RuntimeError if one failed a device-side check.");
+  m.def("device_error_message", [](int32_t bits) { return device_error_text(bits); },
+        pybind11::arg("bits"), "Text for a device error word.");
+  m.def("raise_for_device_error", [](int32_t bits) { raise_for_device_error(bits); },
+        pybind11::arg("bits"), "Raises RuntimeError for a non-zero device error word.");
+  m.def("inject_error_for_testing", &dbscan2d_hip_inject_error_for_testing, pybind11::arg("bits"),
+        "Test hook: the next call on this thread starts with these error bits.");
   m.def("dbscan2d_configured", &dbscan2d_configured, pybind11::arg("X"), pybind11::arg("eps"),
         pybind11::arg("min_samples"), pybind11::arg("config"),
         "dbscan2d with an explicit launch configuration (sort_items, radix_bits, cell_blocks, "
@@ -488,8 +759,10 @@ PYBIND11_MODULE(_C, m) {
   m.def("dbscan2d_batched_out", &dbscan2d_batched_out, pybind11::arg("X"),
         pybind11::arg("offsets"), pybind11::arg("eps"), pybind11::arg("min_samples"),
         pybind11::arg("labels"), pybind11::arg("workspace"),
+        pybind11::arg("error_word") = pybind11::none(),
         "Serving helper: batched DBSCAN into caller-owned device buffers on the current "
-        "stream; allocates nothing.");
+        "stream; allocates nothing. error_word (one int32, device or pinned host memory) "
+        "receives the batch's device error word from its last kernel.");
   m.def("copy_clouds", &copy_clouds, pybind11::arg("dst"), pybind11::arg("srcs"),
         pybind11::arg("rows"),
         "Serving helper: clouds copied into consecutive rows of a device tensor, one "

@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -50,6 +51,8 @@
 namespace {
 
 constexpr int32_t kNoise = -1;
+// Every label of a call whose device-side checks failed (Params::error != 0).
+constexpr int32_t kPoison = INT32_MIN;
 // Radix digits are at most r bits wide (r = the configuration's radix_bits,
 // at most kRadixBitsMax).  A key of b bits is sorted in P = min(ceil(b / r),
 // kMaxPasses) passes of ceil(b / P) bits each, both chosen on the device once
@@ -88,13 +91,15 @@ constexpr uint64_t kCountMask = (1ull << 48) - 1;
 constexpr int kDigitsPerThread = kRadixMax > kSortThreads ? kRadixMax / kSortThreads : 1;
 static_assert(kDigitsPerThread * kSortThreads >= kRadixMax, "every digit has an owner");
 
-// Error bits left in Params::error when a bounded wait gives up (never
-// expected; checked by the tests through the debug layout).
+// Error bits left in Params::error when a bounded wait gives up or an index
+// check fails (never expected).  A call that ends with any bit set writes
+// kPoison to every label; callers read the word with dbscan2d_hip_error_async
+// or dbscan2d_hip_check.
 enum ErrFlag : int32_t {
-  kErrSortSpin = 1,
-  kErrScanSpin = 2,
-  kErrUfLoop = 4,
-  kErrRange = 8  // a computed index fell outside its array (write skipped)
+  kErrSortSpin = DBSCAN_HIP_DEV_SORT_WAIT,
+  kErrScanSpin = DBSCAN_HIP_DEV_SCAN_WAIT,
+  kErrUfLoop = DBSCAN_HIP_DEV_UF_LOOP,
+  kErrRange = DBSCAN_HIP_DEV_RANGE  // a computed index fell outside its array (write skipped)
 };
 
 // First bytes of the workspace.  The first 8 ints are the `info` block of
@@ -349,9 +354,15 @@ __device__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggr
 // indices (dbscan_grid.hpp:80-117), so every walk ends at the component's
 // minimum index.  Loads are agent-scope atomics so that writes by other
 // workgroups (on other XCDs) are seen within the kernel.
-__device__ __forceinline__ int32_t uf_find_ro(const int32_t* uf, int32_t x) {
+// Read-only root walk, bounded by kSpinLimit steps (a longer walk can only
+// come from corrupted parents: kErrUfLoop, and the walk stops there).
+__device__ __forceinline__ int32_t uf_find_ro(const int32_t* uf, int32_t x, int32_t* error) {
   int32_t p = ld_acquire(uf + x);
-  while (p != x) {
+  for (uint32_t it = 0; p != x; ++it) {
+    if (it >= kSpinLimit) {
+      atomicOr(error, static_cast<int32_t>(kErrUfLoop));
+      break;
+    }
     x = p;
     p = ld_acquire(uf + x);
   }
@@ -385,7 +396,8 @@ __device__ void uf_unite(int32_t* uf, int32_t a, int32_t b, int32_t* error) {
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ out, int64_t n,
-                                                  int32_t v) {
+                                                  int32_t v, int32_t* __restrict__ error_out) {
+  if (error_out && blockIdx.x == 0 && threadIdx.x == 0) error_out[0] = 0;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * 256)
     out[i] = v;
@@ -466,7 +478,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const float2* __restrict_
                                               float inv_r, const int4* __restrict__ partials,
                                               int nparts, Params* __restrict__ prm,
                                               int2* __restrict__ col_range, int64_t max_cols,
-                                              int rbits) {
+                                              int rbits, int32_t error0) {
   constexpr int T = kHistThreads;
   __shared__ int32_t s_red[T / 64][4];
   __shared__ uint32_t s_hist[kMaxPasses][kRadixMax];
@@ -482,7 +494,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const float2* __restrict_
     prm->kxmin = v[0]; prm->kxmax = v[1]; prm->kymin = v[2]; prm->kymax = v[3];
     prm->npasses = m.npasses;
     prm->num_cells = 0;
-    prm->error = 0;
+    prm->error = error0;
     prm->col_total = 0;
   }
   for (int e = threadIdx.x; e < kMaxPasses * kRadixMax; e += T) (&s_hist[0][0])[e] = 0u;
@@ -557,7 +569,7 @@ __global__ __launch_bounds__(256) void k_bounds_seg(const float2* __restrict__ X
 __global__ __launch_bounds__(1024) void k_keys_seg(Params* __restrict__ prm,
                                                    CloudInfo* __restrict__ clouds,
                                                    int32_t num_clouds, int64_t max_cols,
-                                                   int rbits) {
+                                                   int rbits, int32_t error0) {
   __shared__ int32_t s_max[16];
   __shared__ uint32_t s_scan[16];
   __shared__ int64_t s_run;
@@ -576,7 +588,7 @@ __global__ __launch_bounds__(1024) void k_keys_seg(Params* __restrict__ prm,
     for (int i = 0; i < 16; ++i) m = max(m, s_max[i]);
     prm->npasses = m;
     prm->num_cells = 0;
-    prm->error = 0;
+    prm->error = error0;
   }
   for (int32_t b0 = 0; b0 < num_clouds; b0 += 1024) {
     const int32_t i = b0 + t;
@@ -716,20 +728,19 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 // cloud's points; the cloud's own histogram, key map and scale apply, the
 // look-back stops at the cloud's first tile, and the cloud's points stay in
 // its own range [begin, end), so every cloud is sorted on its own.
-template <bool kFirst, bool kSeg, int kItems, bool kWalk>
-__device__ __forceinline__ void sort_pass_tiles(
+// One tile, `tile`, of pass `pass` (pass < npasses); the workgroup's last
+// LDS access is followed by a barrier, so a caller may run it in a loop.
+template <bool kFirst, bool kSeg, int kItems>
+__device__ __forceinline__ void sort_pass_tile(
     const float2* __restrict__ X, const float* __restrict__ in_x, const float* __restrict__ in_y,
     const uint32_t* __restrict__ in_i, float* __restrict__ out_x, float* __restrict__ out_y,
     uint32_t* __restrict__ out_i, Params* __restrict__ prm, uint64_t* __restrict__ status,
     int pass, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
     const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist,
-    uint32_t num_tiles) {
+    uint32_t tile) {
   constexpr int kTile = kSortThreads * kItems;
   const uint32_t n = static_cast<uint32_t>(n64);  // n < 2^31: 32-bit offsets
-  if (pass >= prm->npasses) return;
-  // kWalk: a small grid walks the tiles (see launch_sort); otherwise the
-  // workgroup takes tile blockIdx.x only.
-  for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {  // see kTileOrder
+  {
     KeyMap m;
     uint32_t base, tile_n, out_base = 0, first = 0;
     const uint32_t* __restrict__ hist;
@@ -924,7 +935,6 @@ __device__ __forceinline__ void sort_pass_tiles(
       if (pos[q] < n) out_i[pos[q]] = s_idx[t + q * kSortThreads];
     }
     __syncthreads();
-    if constexpr (!kWalk) break;
   }
 }
 
@@ -939,37 +949,28 @@ __global__ __launch_bounds__(kSortThreads, kItems <= 12 ? 4 : 2) void k_sort_pas
     int pass, int64_t n64, float inv_r, const SegTile* __restrict__ tiles,
     const CloudInfo* __restrict__ clouds, const uint32_t* __restrict__ cloud_hist,
     uint32_t num_tiles) {
-  sort_pass_tiles<kFirst, kSeg, kItems, false>(X, in_x, in_y, in_i, out_x, out_y, out_i, prm,
-                                               status, pass, n64, inv_r, tiles, clouds,
-                                               cloud_hist, num_tiles);
+  if (pass >= prm->npasses || blockIdx.x >= num_tiles) return;
+  sort_pass_tile<kFirst, kSeg, kItems>(X, in_x, in_y, in_i, out_x, out_y, out_i, prm, status,
+                                       pass, n64, inv_r, tiles, clouds, cloud_hist,
+                                       blockIdx.x);  // see kTileOrder
 }
 
-// All workgroups of the grid meet here; writes before it are visible to
-// every workgroup after it (agent-scope release and acquire around an atomic
-// count).  Needs every workgroup resident, as the walking passes do; bounded
-// by kSpinLimit.
-__device__ void grid_barrier(uint32_t* counter, uint32_t target, int32_t* error) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(counter, 1u);
-    uint32_t spins = 0;
-    while (ld_acquire(counter) < target) {
-      if (++spins > kSpinLimit) {
-        atomicOr(error, static_cast<int32_t>(kErrSortSpin));
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-    __threadfence();
-  }
-  __syncthreads();
-}
-
-// The radix passes from kTailPass on, all in one launch of one workgroup per
-// CU (they are needed only for keys wider than four digits; for the usual key
-// the launch returns at once): each pass walks the tiles, and a grid barrier
-// separates the passes.  Pass p reads buffer set a if p is odd, else b.
+// The radix passes from kTailPass on (needed only for keys wider than four
+// digits; for the usual key the launch returns at once), in one launch that
+// needs no workgroup to be resident beside another: a workgroup claims work
+// items by an atomic ticket, item k being tile k mod num_tiles of pass
+// kTailPass + k / num_tiles, so every tile of a pass is claimed after all
+// tiles of the passes before it and after the earlier tiles of its own pass.
+// A tile's look-back waits only on earlier tiles of its pass, claimed by
+// workgroups that are running; a pass's first tiles wait until every tile of
+// the pass before has finished (a done counter per pass, agent-scope fence
+// before the count and after the wait, which also invalidates this CU's
+// vector cache before the earlier pass's output is read).  A workgroup holds
+// one tile at a time and finishes it before it claims the next, so the
+// claimed tiles always make progress whatever the residency.  Every wait is
+// bounded by kSpinLimit.  Pass p reads buffer set a if p is odd, else b.
+// tile_counter[0] is the ticket, tile_counter[1 + q] the tiles done of pass
+// kTailPass + q (cleared by the call's first kernel).
 template <bool kSeg, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
     float* __restrict__ xs_a, float* __restrict__ ys_a, uint32_t* __restrict__ idx_a,
@@ -977,16 +978,57 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
     Params* __restrict__ prm, uint64_t* __restrict__ status, int64_t n64, float inv_r,
     const SegTile* __restrict__ tiles, const CloudInfo* __restrict__ clouds,
     const uint32_t* __restrict__ cloud_hist, uint32_t num_tiles) {
+  static_assert(kMaxPasses - kTailPass + 1 <= 16, "one done counter per tail pass");
   const int np = prm->npasses;
-  for (int pass = kTailPass; pass < np; ++pass) {
+  if (np <= kTailPass || num_tiles == 0) return;
+  const uint32_t items = static_cast<uint32_t>(np - kTailPass) * num_tiles;
+  __shared__ uint32_t s_ticket;
+  // Every branch around the barriers below is wave-uniform: wave 0 as a
+  // whole claims the ticket, waits and counts (lane 0 contributes the only
+  // increment), the others skip those blocks.  A branch on threadIdx.x == 0
+  // at the end of the body and another at its start were merged by the
+  // compiler across the loop's back edge into a divergent exit, which let
+  // the other lanes run into the next iteration's barrier ahead of thread 0.
+  const bool wave0 = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x)) == 0;
+  const uint32_t one = threadIdx.x == 0 ? 1u : 0u;
+  for (;;) {
+    if (wave0) {
+      const uint32_t old = atomicAdd(&prm->tile_counter[0], one);
+      s_ticket = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(old), 0));
+    }
+    __syncthreads();
+    const uint32_t k = static_cast<uint32_t>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(s_ticket)));
+    __syncthreads();
+    if (k >= items) break;
+    const int q = static_cast<int>(k / num_tiles);
+    const int pass = kTailPass + q;
+    const uint32_t tile = k - static_cast<uint32_t>(q) * num_tiles;
+    if (q > 0) {
+      if (wave0) {
+        uint32_t spins = 0;
+        while (static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(
+                   ld_acquire(&prm->tile_counter[q])))) < num_tiles) {
+          if (++spins > kSpinLimit) {
+            atomicOr(&prm->error, static_cast<int32_t>(kErrSortSpin));
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      __syncthreads();
+      __threadfence();  // acquire in every wave before the earlier pass's output is read
+    }
     const bool from_a = (pass & 1) != 0;
-    sort_pass_tiles<false, kSeg, kItems, true>(
+    sort_pass_tile<false, kSeg, kItems>(
         nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b, from_a ? idx_a : idx_b,
         from_a ? xs_b : xs_a, from_a ? ys_b : ys_a, from_a ? idx_b : idx_a, prm, status, pass,
-        n64, inv_r, tiles, clouds, cloud_hist, num_tiles);
-    if (pass + 1 < np)
-      grid_barrier(&prm->tile_counter[0], gridDim.x * static_cast<uint32_t>(pass - kTailPass + 1),
-                   &prm->error);
+        n64, inv_r, tiles, clouds, cloud_hist, tile);
+    // every thread's writes of the tile are made visible before it counts
+    // as done
+    __threadfence();
+    __syncthreads();
+    if (wave0) atomicAdd(&prm->tile_counter[q + 1], one);
   }
 }
 
@@ -2136,7 +2178,8 @@ __global__ __launch_bounds__(256) void k_union(
       const int32_t cw = bcast(c, src), nw = bcast(nc, src), sw = bcast(slot, src);
       const float weps = bcastf(eps_sq, src);
       const int32_t rc = cell_ref[cw], rn = cell_ref[nw];
-      if (kPhase == 1 && uf_find_ro(uf, rc) == uf_find_ro(uf, rn)) continue;
+      if (kPhase == 1 && uf_find_ro(uf, rc, &prm->error) == uf_find_ro(uf, rn, &prm->error))
+        continue;
       const bool hit = core_pair_within(xs, ys, is_core, cell_start[cw], cell_start[cw + 1],
                                         load_box(cell_box, cw), cell_start[nw],
                                         cell_start[nw + 1], load_box(cell_box, nw), weps);
@@ -2259,7 +2302,7 @@ __global__ __launch_bounds__(kUnionLocalThreads) void k_union_local(
 // uf[r] = root by a read-only walk (each thread writes only its own slot; the
 // reference's determinism fix, csrc/cuda/dbscan_cuda.cu:288-299,372-378), and
 // the root flag of each reference index.
-__global__ __launch_bounds__(256) void k_flatten(const Params* __restrict__ prm,
+__global__ __launch_bounds__(256) void k_flatten(Params* __restrict__ prm,
                                                  const int32_t* __restrict__ cell_ref,
                                                  const uint32_t* __restrict__ cell_flags,
                                                  int32_t* __restrict__ uf,
@@ -2268,7 +2311,7 @@ __global__ __launch_bounds__(256) void k_flatten(const Params* __restrict__ prm,
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < C;
        c += static_cast<int64_t>(gridDim.x) * 256) {
     const int32_t r = cell_ref[c];
-    const int32_t root = uf_find_ro(uf, r);
+    const int32_t root = uf_find_ro(uf, r, &prm->error);
     core_ref[r] = cell_flags[c] & 1u;  // the cell's core bit, by reference index
     st_release(uf + r, root);
   }
@@ -2328,7 +2371,7 @@ __global__ __launch_bounds__(kScanThreads) void k_rank(Params* __restrict__ prm,
 // kSeg (batches): ids count from 0 in every cloud, so the number of roots
 // before the cloud's first reference index is subtracted.
 template <bool kSeg>
-__global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
+__global__ __launch_bounds__(256) void k_cluster(Params* __restrict__ prm,
                                                  const int32_t* __restrict__ cell_ref,
                                                  const uint32_t* __restrict__ cell_flags,
                                                  int32_t* __restrict__ uf,
@@ -2343,7 +2386,7 @@ __global__ __launch_bounds__(256) void k_cluster(const Params* __restrict__ prm,
     // the root by a read-only walk, written back (uf ends flattened; every
     // value written is a root, so concurrent walks stay on valid ancestors)
     const int32_t r = cell_ref[c];
-    const int32_t root = uf_find_ro(uf, r);
+    const int32_t root = uf_find_ro(uf, r, &prm->error);
     if (root != r) st_release(uf + r, root);
     if (cell_flags[c] & 1u) {
       id = static_cast<int32_t>(rank[root]);
@@ -2504,7 +2547,9 @@ __global__ __launch_bounds__(256) void k_label_direct(
     const uint32_t* __restrict__ idx_b, const int32_t* __restrict__ point_cell,
     const uint8_t* __restrict__ is_core, const int32_t* __restrict__ cell_cluster,
     const uint32_t* __restrict__ spare_ia, const uint32_t* __restrict__ spare_ib,
-    int32_t* __restrict__ labels, int64_t n) {
+    int32_t* __restrict__ labels, int64_t n, int32_t* __restrict__ error_out) {
+  // the last kernel of a call: the error word is final (this kernel sets no bit)
+  if (error_out && blockIdx.x == 0 && threadIdx.x == 0) error_out[0] = prm->error;
   // Workgroups go to the 8 XCDs in turn; each XCD takes a contiguous eighth of
   // the sorted points.  In a batch a cloud's points stay inside its own input
   // range, so one cloud's scattered label writes then gather in one XCD's L2
@@ -2518,7 +2563,8 @@ __global__ __launch_bounds__(256) void k_label_direct(
   const int32_t* __restrict__ lab_sorted =
       reinterpret_cast<const int32_t*>(in_a ? spare_ib : spare_ia);
   if (dst < static_cast<uint64_t>(n))
-    labels[dst] = is_core[i] ? cell_cluster[point_cell[i]] : lab_sorted[i];
+    labels[dst] = prm->error != 0 ? kPoison
+                                  : (is_core[i] ? cell_cluster[point_cell[i]] : lab_sorted[i]);
 }
 
 // Pass B: one workgroup per bucket reads the bucket's run in every tile's
@@ -2631,7 +2677,9 @@ __global__ __launch_bounds__(kThreads) void k_unpermute(
     const uint32_t* __restrict__ spare_xb, const uint32_t* __restrict__ spare_ya,
     const uint32_t* __restrict__ spare_yb, const uint32_t* __restrict__ table,
     const uint32_t* __restrict__ long_runs, int32_t* __restrict__ labels, int64_t n64,
-    uint32_t B) {
+    uint32_t B, int32_t* __restrict__ error_out) {
+  // the last kernel of a call: the error word is final (this kernel sets no bit)
+  if (error_out && blockIdx.x == 0 && threadIdx.x == 0) error_out[0] = prm->error;
   constexpr uint32_t W = kThreads / 64;  // waves
   constexpr uint32_t G = kThreads / 16;  // groups; group g reads the runs of tiles g, g + G, ...
   static_assert((W & (W - 1)) == 0, "waves per workgroup a power of two");
@@ -2714,7 +2762,9 @@ __global__ __launch_bounds__(kThreads) void k_unpermute(
     }
   }
   __syncthreads();
-  for (uint32_t j = b0 + t; j < b1; j += kThreads) labels[j] = s_lab[j - b0];
+  // a call whose device-side checks failed leaves every label kPoison
+  const bool failed = prm->error != 0;
+  for (uint32_t j = b0 + t; j < b1; j += kThreads) labels[j] = failed ? kPoison : s_lab[j - b0];
 }
 
 // Border search for one point by one lane (dbscan_grid.hpp:419-445): over
@@ -2929,6 +2979,13 @@ __global__ __launch_bounds__(256) void k_border(
   }
 }
 
+// The error word of the call that last used the workspace, to dst (device
+// memory or pinned host memory), in stream order after that call.
+__global__ __launch_bounds__(64) void k_copy_error(const Params* __restrict__ prm,
+                                                   int32_t* __restrict__ dst) {
+  if (threadIdx.x == 0) dst[0] = prm->error;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -3000,6 +3057,15 @@ Layout make_layout(int64_t n, int64_t clouds = 0) {
 }
 
 thread_local int g_last_hip_error = 0;
+// Error bits the next call on this host thread starts with (test hook:
+// dbscan2d_hip_inject_error_for_testing); consumed by that call.
+thread_local int32_t g_injected_error = 0;
+
+int32_t take_injected_error() {
+  const int32_t v = g_injected_error;
+  g_injected_error = 0;
+  return v;
+}
 
 const char* const kStageNames[DBSCAN_HIP_NUM_STAGES] = {
     "cell_keys", "sort_gather", "boundary_scan", "build_ngh", "core_mark",
@@ -3016,16 +3082,16 @@ uint32_t bucket_width(int64_t n) {
 
 void launch_unpermute(uint32_t B, int64_t n, hipStream_t stream, const Params* prm,
                       float* xs_a, float* xs_b, float* ys_a, float* ys_b, const uint64_t* status,
-                      const uint32_t* long_runs, int32_t* labels) {
+                      const uint32_t* long_runs, int32_t* labels, int32_t* error_out) {
   const int nb = static_cast<int>((n + B - 1) / B);
   auto u = [](float* p) { return reinterpret_cast<uint32_t*>(p); };
   const uint32_t* table = reinterpret_cast<const uint32_t*>(status);
   if (B <= static_cast<uint32_t>(kUnpermSmallWidth))
     k_unpermute<kUnpermSmallWidth, 512><<<nb, 512, 0, stream>>>(
-        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), table, long_runs, labels, n, B);
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), table, long_runs, labels, n, B, error_out);
   else
     k_unpermute<kUnpermMaxWidth, 1024><<<nb, 1024, 0, stream>>>(
-        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), table, long_runs, labels, n, B);
+        prm, u(xs_a), u(xs_b), u(ys_a), u(ys_b), table, long_runs, labels, n, B, error_out);
 }
 
 int grid_for(int64_t items, int per_block, int cap);
@@ -3110,21 +3176,29 @@ bool config_ok(const dbscan_hip_config& c) {
 
 // Passes from kTailPass on are needed only for keys wider than 4 digits of
 // the configuration's radix_bits (36 bits at 9); they run in one launch
-// (k_sort_tail) of one workgroup per CU that walks the tiles, so that for the
-// usual narrow key they cost a single launch that returns at once.  One workgroup per CU is always
-// co-resident, which the walk and the barrier between passes need: a
-// workgroup waits only on tiles of lower index, held by workgroups that are
-// resident or done.  (The walk is a separate instantiation: its loop would
-// cost the one-tile kernel registers.)
+// (k_sort_tail) whose workgroups claim tiles by ticket, so that for the usual
+// narrow key they cost a single launch that returns at once.  Its grid is two
+// workgroups per CU (any size is correct: no workgroup waits for one that has
+// not claimed a tile).  The CU count is read once per device.
+constexpr int kMaxDevices = 64;
+std::atomic<int> g_cu_count[kMaxDevices];
 
-int tail_grid(int tiles) {
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+int cu_count() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 1;
+  if (dev < kMaxDevices) {
+    const int c = g_cu_count[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus < 1)
-    cus = 1;
-  return std::min(tiles, cus);
+    return 1;
+  if (dev < kMaxDevices) g_cu_count[dev].store(cus, std::memory_order_relaxed);
+  return cus;
 }
+
+int tail_grid(int tiles) { return std::max(1, std::min(tiles, 2 * cu_count())); }
 
 // Radix pass `pass` with `items` points per thread over `tiles` tiles.
 template <bool kFirst, bool kSeg, class... A>
@@ -3153,7 +3227,7 @@ void launch_sort_tail(int items, int tiles, hipStream_t st, A... a) {
 
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
         void* workspace, size_t workspace_bytes, hipStream_t stream, float* stage_ms,
-        hipEvent_t* kev, const dbscan_hip_config* cfgp) {
+        hipEvent_t* kev, const dbscan_hip_config* cfgp, int32_t* error_out = nullptr) {
   if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
   if (!(eps > 0.0)) return DBSCAN_HIP_ERR_EPS;  // csrc/extension.cpp:29
   if (min_samples64 < 1) return DBSCAN_HIP_ERR_MIN_SAMPLES;  // csrc/extension.cpp:30
@@ -3206,7 +3280,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   if (ms < 1) {  // dbscan_grid.hpp:141: every point is noise
     for (int k = 0; k < kNumKernels; ++k)
       if ((e = before(k)) != hipSuccess) return fail(e);
-    k_fill_i32<<<grid_for(n, 256, 4096), 256, 0, stream>>>(labels, n, kNoise);
+    k_fill_i32<<<grid_for(n, 256, 4096), 256, 0, stream>>>(labels, n, kNoise, error_out);
     if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     if (stage_ms && (e = hipEventSynchronize(ev[kNumKernels])) != hipSuccess) return fail(e);
@@ -3253,7 +3327,8 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
                                          at<uint4>(ws, kParamsZeroBegin), zero16);
   if ((e = before(kKHist)) != hipSuccess) return fail(e);
   k_hist<<<g_hist, kHistThreads, 0, stream>>>(X2, n, inv_r, at<int4>(ws, L.partials), g_reduce, prm,
-                                     at<int2>(ws, L.col_range), L.max_cols, cfg.radix_bits);
+                                     at<int2>(ws, L.col_range), L.max_cols, cfg.radix_bits,
+                                              take_injected_error());
 
   // Stage 2: radix passes (sort + gather in one).
   if ((e = before(kKSort0)) != hipSuccess) return fail(e);
@@ -3329,12 +3404,12 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
                  ys_a, ys_b, sort_status, at<uint32_t>(ws, L.long_runs));
   } else {
     k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
-        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
+        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n, error_out);
   }
   if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
   if (bw > 0)
     launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, sort_status,
-                     at<uint32_t>(ws, L.long_runs), labels);
+                     at<uint32_t>(ws, L.long_runs), labels, error_out);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
 
@@ -3368,7 +3443,7 @@ bool cloud_params(double eps, int64_t min_samples64, CloudInfo& ci) {
 int run_batched(const float* X, int64_t total, const int64_t* offsets, const double* eps,
                 const int64_t* min_samples, int64_t num_clouds, int32_t* labels,
                 void* workspace, size_t workspace_bytes, hipStream_t stream, hipEvent_t* kev,
-                int* bad_cloud, const dbscan_hip_config* cfgp) {
+                int* bad_cloud, const dbscan_hip_config* cfgp, int32_t* error_out = nullptr) {
   if (bad_cloud) *bad_cloud = -1;
   const dbscan_hip_config cfg = cfgp ? *cfgp : default_config(total);
   if (!config_ok(cfg)) return DBSCAN_HIP_ERR_CONFIG;
@@ -3481,7 +3556,8 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const int64_t zero16 = static_cast<int64_t>((L.zero_end - kParamsZeroBegin) / 16);
   k_bounds_seg<<<g_sort, 256, 0, stream>>>(X2, seg_sort, clouds, at<uint4>(ws, kParamsZeroBegin),
                                            zero16);
-  k_keys_seg<<<1, 1024, 0, stream>>>(prm, clouds, B, L.max_cols, cfg.radix_bits);
+  k_keys_seg<<<1, 1024, 0, stream>>>(prm, clouds, B, L.max_cols, cfg.radix_bits,
+                                     take_injected_error());
   if ((e = before(kKHist)) != hipSuccess) return fail(e);
   k_hist_seg<<<g_hist, 256, 0, stream>>>(X2, seg_sort, g_sort, hist_per_block, clouds, prm,
                                          cloud_hist, col_range);
@@ -3545,12 +3621,12 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
                  ys_a, ys_b, sort_status, at<uint32_t>(ws, L.long_runs));
   } else {
     k_label_direct<<<grid_for(n, 256, INT_MAX), 256, 0, stream>>>(
-        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n);
+        prm, idx_a, idx_b, point_cell, is_core, cell_cluster, idx_a, idx_b, labels, n, error_out);
   }
   if ((e = before(kKUnpermute)) != hipSuccess) return fail(e);
   if (bw > 0)
     launch_unpermute(bw, n, stream, prm, xs_a, xs_b, ys_a, ys_b, sort_status,
-                     at<uint32_t>(ws, L.long_runs), labels);
+                     at<uint32_t>(ws, L.long_runs), labels, error_out);
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   return DBSCAN_HIP_OK;
@@ -3584,6 +3660,13 @@ int dbscan2d_hip_configured(const float* X, int64_t n, double eps, int64_t min_s
                             hipStream_t stream, const dbscan_hip_config* cfg) {
   return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
              nullptr, cfg);
+}
+
+int dbscan2d_hip_checked(const float* X, int64_t n, double eps, int64_t min_samples,
+                         int32_t* labels, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream, const dbscan_hip_config* cfg, int32_t* error_out) {
+  return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
+             nullptr, cfg, error_out);
 }
 
 int dbscan2d_hip_timed(const float* X, int64_t n, double eps, int64_t min_samples,
@@ -3628,6 +3711,15 @@ int dbscan2d_hip_batched(const float* X, int64_t total_points, const int64_t* of
                      workspace_bytes, stream, nullptr, bad_cloud, cfg);
 }
 
+int dbscan2d_hip_batched_checked(const float* X, int64_t total_points, const int64_t* offsets,
+                                 const double* eps, const int64_t* min_samples,
+                                 int64_t num_clouds, int32_t* labels, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream, int* bad_cloud,
+                                 const dbscan_hip_config* cfg, int32_t* error_out) {
+  return run_batched(X, total_points, offsets, eps, min_samples, num_clouds, labels, workspace,
+                     workspace_bytes, stream, nullptr, bad_cloud, cfg, error_out);
+}
+
 int dbscan2d_hip_batched_timed(const float* X, int64_t total_points, const int64_t* offsets,
                                const double* eps, const int64_t* min_samples,
                                int64_t num_clouds, int32_t* labels, void* workspace,
@@ -3667,10 +3759,53 @@ const char* dbscan2d_hip_status_string(int status) {
     case DBSCAN_HIP_ERR_ALIGN: return "misaligned pointer (X 8 B, labels 4 B, workspace 256 B)";
     case DBSCAN_HIP_ERR_HIP: return "HIP runtime error";
     case DBSCAN_HIP_ERR_CONFIG: return "invalid launch configuration";
+    case DBSCAN_HIP_ERR_DEVICE:
+      return "a device-side check failed; every label of the call is INT32_MIN";
     default: return "unknown status";
   }
 }
 
 int dbscan2d_hip_last_hip_error(void) { return g_last_hip_error; }
+
+int dbscan2d_hip_error_async(const void* workspace, int32_t* dst, hipStream_t stream) {
+  if (!workspace || !dst) return DBSCAN_HIP_ERR_NULL;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255u) return DBSCAN_HIP_ERR_ALIGN;
+  k_copy_error<<<1, 64, 0, stream>>>(static_cast<const Params*>(workspace), dst);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  }
+  return DBSCAN_HIP_OK;
+}
+
+int dbscan2d_hip_check(const void* workspace, hipStream_t stream, int32_t* bits) {
+  if (bits) *bits = 0;
+  if (!workspace) return DBSCAN_HIP_ERR_NULL;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255u) return DBSCAN_HIP_ERR_ALIGN;
+  int32_t word = 0;
+  hipError_t e = hipStreamSynchronize(stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(&word, static_cast<const char*>(workspace) + offsetof(Params, error),
+                  sizeof(word), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  }
+  if (bits) *bits = word;
+  return word ? DBSCAN_HIP_ERR_DEVICE : DBSCAN_HIP_OK;
+}
+
+const char* dbscan2d_hip_device_error_name(int32_t bit) {
+  switch (bit) {
+    case DBSCAN_HIP_DEV_SORT_WAIT: return "a radix-sort look-back or pass wait timed out";
+    case DBSCAN_HIP_DEV_SCAN_WAIT: return "a scan look-back wait timed out";
+    case DBSCAN_HIP_DEV_UF_LOOP: return "a union-find operation reached its step limit";
+    case DBSCAN_HIP_DEV_RANGE: return "a computed index fell outside its array";
+    default: return nullptr;
+  }
+}
+
+void dbscan2d_hip_inject_error_for_testing(int32_t bits) { g_injected_error = bits; }
 
 }  // extern "C"

@@ -58,7 +58,7 @@ _load_tuning()
 
 ArrayLike2D = Union[np.ndarray, torch.Tensor]
 
-__all__ = ["dbscan2d", "dbscan2d_batched", "DBSCAN", "ArrayLike2D"]
+__all__ = ["dbscan2d", "dbscan2d_batched", "DBSCAN", "ArrayLike2D", "check_device_errors"]
 
 try:
     __version__ = _pkg_version("dbscan-torch")
@@ -67,22 +67,28 @@ except PackageNotFoundError:
 
 
 def dbscan2d(X: torch.Tensor, eps: float, min_samples: int) -> torch.Tensor:
-    """Run DBSCAN on a 2D point set.
+    """Cluster the rows of a float32 ``(N, 2)`` tensor with grid-hashed DBSCAN.
 
-    The output tensor is on the same device as the input.  ROCm tensors are
-    clustered in place on their device with no host round trip; host tensors
-    are staged to the current HIP device and the labels copied back.
+    ``eps`` is the neighbourhood radius (must be > 0); a point is a core point
+    when at least ``min_samples`` points (itself included) lie within
+    ``eps``.  The result is an int32 ``(N,)`` tensor on X's device: ``-1`` for
+    noise, else a cluster id in ``[0, n_clusters)`` numbered as the
+    reference CPU path numbers them.  A ROCm tensor is clustered where it
+    lies and the call returns without waiting for the GPU; a host tensor is
+    copied to the current HIP device and its labels come back to the host.
 
-    Args:
-        X: float32 tensor of shape (N, 2).
-        eps: neighborhood radius (> 0).
-        min_samples: minimum neighbors (incl. self) to be a core point (>= 1).
-
-    Returns:
-        int32 tensor of shape (N,). -1 indicates noise, otherwise a cluster id
-        in ``[0, n_clusters)``, numbered as the reference CPU path numbers them.
+    A failed device-side check (never expected) leaves every label INT32_MIN
+    and raises RuntimeError: at once for a host tensor, else from the next
+    call on the device or from :func:`check_device_errors`.
     """
     return _C.dbscan2d(X, float(eps), int(min_samples))
+
+
+def check_device_errors(device: Optional[int] = None) -> None:
+    """Wait for every dbscan2d / dbscan2d_batched call that returned device
+    labels without waiting, and raise RuntimeError if one of them failed a
+    device-side check (a bounded wait that gave up, an index out of range)."""
+    _C.check_device_errors(device)
 
 
 def _per_cloud(value, count: int, name: str, cast):
@@ -143,16 +149,12 @@ def dbscan2d_batched(clouds: Union[Sequence[torch.Tensor], torch.Tensor],
 
 
 class DBSCAN:
-    """sklearn-compatible class wrapper around :func:`dbscan2d`
-    (reference dbscan_torch/__init__.py:76-118).
+    """Estimator-style front end to :func:`dbscan2d` with the reference's
+    constructor defaults and methods (dbscan_torch/__init__.py:76-118).
 
-    Input type is preserved end-to-end: ``numpy.ndarray`` in ->
-    ``numpy.ndarray`` out; ``torch.Tensor`` in -> ``torch.Tensor`` out on the
-    same device.  Non-float32 input is cast to float32.
-
-    Args:
-        eps: neighborhood radius (> 0).
-        min_samples: minimum neighbors (incl. self) to be a core point (>= 1).
+    ``fit`` stores the labels in ``labels_`` in the container type it was
+    given (a numpy array for numpy input, a tensor on the input's device for
+    a tensor); input of another dtype is converted to float32 first.
     """
 
     def __init__(self, eps: float = 0.5, min_samples: int = 5):
@@ -161,7 +163,7 @@ class DBSCAN:
         self.labels_: Optional[ArrayLike2D] = None
 
     def fit(self, X: ArrayLike2D, y: Any = None) -> "DBSCAN":
-        """Fit the clusterer on ``X``; labels go to ``self.labels_``.  ``y`` is ignored."""
+        """Cluster ``X`` and keep its labels in ``labels_`` (``y`` is unused)."""
         is_torch = isinstance(X, torch.Tensor)
         if is_torch:
             Xt = X if X.dtype == torch.float32 else X.to(torch.float32)
@@ -172,5 +174,5 @@ class DBSCAN:
         return self
 
     def fit_predict(self, X: ArrayLike2D, y: Any = None) -> ArrayLike2D:
-        """Fit and return labels in one call."""
+        """``fit(X)`` followed by returning ``labels_``."""
         return self.fit(X, y).labels_  # type: ignore[return-value]

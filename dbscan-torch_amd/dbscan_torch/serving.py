@@ -43,7 +43,9 @@ class Request:
     """One dbscan2d call.  ``X`` is a float32 (N, 2) tensor, on the host
     (pinned memory lets its copy overlap) or on the server's device.
     ``arrival`` is the time the request is due (clock seconds); latency is
-    measured from it to the moment its labels are in host memory."""
+    measured from it to the moment its labels are in host memory.  ``error``
+    is None, or the text of a failed device-side check of its batch (its
+    labels are then INT32_MIN)."""
 
     rid: int
     X: torch.Tensor
@@ -53,6 +55,7 @@ class Request:
     labels: Optional[torch.Tensor] = None
     admitted: Optional[float] = None
     done: Optional[float] = None
+    error: Optional[str] = None
 
     @property
     def n(self) -> int:
@@ -78,6 +81,8 @@ class _Slot:
         self.workspace = arena.narrow(0, offset + x_bytes + l_bytes, ws_bytes)
         self.stream = torch.cuda.Stream(device)  # default priority
         self.event: Optional[torch.cuda.Event] = None
+        # the batch's device error word, copied back with its labels
+        self.err = torch.zeros(1, dtype=torch.int32, pin_memory=True)
         self.batch: List[Request] = []
         self.offsets: List[int] = []
         self.host_labels: Optional[torch.Tensor] = None
@@ -121,6 +126,7 @@ class Server:
         self.free: Deque[_Slot] = collections.deque(self.slots)
         self.inflight: Deque[_Slot] = collections.deque()
         self.completed: Dict[int, Request] = {}
+        self.failed: List[int] = []  # ids of requests whose batch failed a device check
         self.batches: List[int] = []  # requests per launched batch
 
     def footprint_bytes(self, n: int) -> int:
@@ -163,11 +169,19 @@ class Server:
         for r in batch:
             r.admitted = now
         host = torch.empty(pts, dtype=torch.int32, pin_memory=True)
+        # device-resident inputs may still be being written on the caller's
+        # stream: the slot's stream waits for it, and their memory is kept
+        # until the slot's copies have read it
+        if any(r.X.is_cuda for r in batch):
+            slot.stream.wait_stream(torch.cuda.current_stream(self.device))
+            for r in batch:
+                if r.X.is_cuda:
+                    r.X.record_stream(slot.stream)
         with torch.cuda.stream(slot.stream):
             _C.copy_clouds(slot.X, [r.X for r in batch], offs[:-1])
             _C.dbscan2d_batched_out(slot.X[:pts], offs, [r.eps for r in batch],
                                     [r.min_samples for r in batch], slot.labels,
-                                    slot.workspace)
+                                    slot.workspace, slot.err)
             host.copy_(slot.labels[:pts], non_blocking=True)
             slot.event = torch.cuda.Event()
             slot.event.record(slot.stream)
@@ -177,9 +191,14 @@ class Server:
 
     def _retire(self, slot: _Slot) -> None:
         now = self.clock()
+        bits = int(slot.err[0])
+        why = _C.device_error_message(bits) if bits else None
         for r, o in zip(slot.batch, slot.offsets):
             r.labels = slot.host_labels[o:o + r.n]
             r.done = now
+            r.error = why
+            if why:
+                self.failed.append(r.rid)
             self.completed[r.rid] = r
         slot.batch, slot.offsets, slot.host_labels, slot.event = [], [], None, None
         self.free.append(slot)

@@ -33,8 +33,26 @@ enum {
   DBSCAN_HIP_ERR_WORKSPACE = 5,      /* workspace smaller than the required bytes  */
   DBSCAN_HIP_ERR_ALIGN = 6,          /* X, labels or workspace misaligned          */
   DBSCAN_HIP_ERR_HIP = 7,            /* a HIP runtime call failed (see last error) */
-  DBSCAN_HIP_ERR_CONFIG = 8          /* invalid dbscan_hip_config                  */
+  DBSCAN_HIP_ERR_CONFIG = 8,         /* invalid dbscan_hip_config                  */
+  DBSCAN_HIP_ERR_DEVICE = 9          /* a device-side check failed (dbscan2d_hip_check) */
 };
+
+/*
+ * Device-side error bits.  Every bounded wait inside the kernels (look-back
+ * spins, the sort's pass hand-off) and the union-find step loop gives up
+ * after a fixed number of steps instead of hanging, and every computed index
+ * is range-checked before a write; a failure sets a bit in the workspace's
+ * error word.  None is expected on valid input.  A call that ends with any
+ * bit set writes INT32_MIN to every label (so no failed result passes for a
+ * valid one), and the word is read back with dbscan2d_hip_error_async or
+ * dbscan2d_hip_check.  The reference has no counterpart (its kernels have no
+ * bounded waits; it checks cudaGetLastError after its launches,
+ * csrc/cuda/dbscan_cuda.cu:738).
+ */
+#define DBSCAN_HIP_DEV_SORT_WAIT 1
+#define DBSCAN_HIP_DEV_SCAN_WAIT 2
+#define DBSCAN_HIP_DEV_UF_LOOP 4
+#define DBSCAN_HIP_DEV_RANGE 8
 
 /*
  * Launch configuration (no counterpart in the reference, whose launch shapes
@@ -89,6 +107,17 @@ int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples,
 int dbscan2d_hip_configured(const float* X, int64_t n, double eps, int64_t min_samples,
                             int32_t* labels, void* workspace, size_t workspace_bytes,
                             hipStream_t stream, const dbscan_hip_config* cfg);
+
+/*
+ * dbscan2d_hip_configured that also reports the call's device error word:
+ * its last kernel writes the word to *error_out (device memory or pinned
+ * host memory; NULL: not written), in stream order, with no extra launch.
+ * Not written when n == 0 (nothing is enqueued).  No counterpart in the
+ * reference; the binding uses it to check calls it does not wait for.
+ */
+int dbscan2d_hip_checked(const float* X, int64_t n, double eps, int64_t min_samples,
+                         int32_t* labels, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream, const dbscan_hip_config* cfg, int32_t* error_out);
 
 /*
  * Same as dbscan2d_hip, and additionally writes the GPU time of each of the
@@ -147,6 +176,14 @@ int dbscan2d_hip_batched(const float* X, int64_t total_points, const int64_t* of
                          int32_t* labels, void* workspace, size_t workspace_bytes,
                          hipStream_t stream, int* bad_cloud, const dbscan_hip_config* cfg);
 
+/* dbscan2d_hip_batched that also reports the batch's device error word, as
+ * dbscan2d_hip_checked does.  No counterpart in the reference. */
+int dbscan2d_hip_batched_checked(const float* X, int64_t total_points, const int64_t* offsets,
+                                 const double* eps, const int64_t* min_samples,
+                                 int64_t num_clouds, int32_t* labels, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream, int* bad_cloud,
+                                 const dbscan_hip_config* cfg, int32_t* error_out);
+
 /*
  * dbscan2d_hip_batched with the event slots of dbscan2d_hip_timed (slot 0
  * also covers the per-cloud key-map kernel).  No counterpart in the
@@ -190,6 +227,34 @@ const char* dbscan2d_hip_status_string(int status);
 
 /* The hipError_t behind the most recent DBSCAN_HIP_ERR_HIP on this thread. */
 int dbscan2d_hip_last_hip_error(void);
+
+/*
+ * Enqueues on `stream` a copy of the error word that the last call using
+ * `workspace` (single or batched) leaves, into *dst: device memory or pinned
+ * (hipHostMalloc) host memory, valid once the stream has passed this point.
+ * One one-thread kernel; no synchronisation.  Pairs with the C10_CUDA_CHECK /
+ * cudaGetLastError checks of the reference binding (csrc/extension.cpp:36-42,
+ * csrc/cuda/dbscan_cuda.cu:738), which see launch errors only.
+ */
+int dbscan2d_hip_error_async(const void* workspace, int32_t* dst, hipStream_t stream);
+
+/*
+ * Waits for `stream`, then reads the error word of the last call that used
+ * `workspace`: DBSCAN_HIP_OK if it is 0, else DBSCAN_HIP_ERR_DEVICE with the
+ * word in *bits (if not NULL).
+ */
+int dbscan2d_hip_check(const void* workspace, hipStream_t stream, int32_t* bits);
+
+/* Text for one device-side error bit (DBSCAN_HIP_DEV_*), or NULL. */
+const char* dbscan2d_hip_device_error_name(int32_t bit);
+
+/*
+ * Test hook (no counterpart in the reference): the next call on this host
+ * thread that runs the pipeline starts with `bits` in its error word, so the
+ * reporting path (poisoned labels, error word, exceptions) can be tested
+ * without making any kernel fail.  Consumed by that call.
+ */
+void dbscan2d_hip_inject_error_for_testing(int32_t bits);
 
 #ifdef __cplusplus
 }  /* extern "C" */

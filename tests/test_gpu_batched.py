@@ -158,6 +158,36 @@ def test_clouds2048_sample_equals_single_calls(dt):
         assert np.array_equal(got[i], oracle.dbscan2d(clouds[i], e[i], m[i])), i
 
 
+def _digest(labels):
+    return hashlib.sha256(np.ascontiguousarray(labels, dtype=np.int32).tobytes()).hexdigest()[:16]
+
+
+@pytest.mark.timeout(900)
+def test_clouds2048_full_batch_matches_reference_digests(dt, golden_dir):
+    """All 2,048 clouds of the batched workload (135M points, per-cloud N
+    log-uniform in [200, 500,000], own eps and min_samples) in one batched
+    call: every cloud's labels equal the compiled reference's
+    (tests/golden/streams.json, digests of its int32 labels)."""
+    with open(os.path.join(golden_dir, "streams.json")) as f:
+        want = json.load(f)["clouds2048"]
+    ns, eps, ms = W.clouds2048_params()
+    assert int(ns.sum()) == want["points"]
+    offs = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    X = np.empty((int(offs[-1]), 2), np.float32)
+    for i in range(len(ns)):
+        X[offs[i]:offs[i + 1]] = W.clouds2048_cloud(i, int(ns[i]))
+        if i % 512 == 511:
+            print(f"[gpu batch] clouds2048: {i + 1} clouds generated", flush=True)
+    Xd = torch.from_numpy(X).to(DEV)
+    del X
+    print(f"[gpu batch] clouds2048: one batched call on {int(offs[-1])} points", flush=True)
+    labels = dt.dbscan2d_batched(Xd, [float(e) for e in eps], [int(m) for m in ms],
+                                 offsets=offs.tolist()).cpu().numpy()
+    dt.check_device_errors()
+    bad = [i for i in range(len(ns)) if _digest(labels[offs[i]:offs[i + 1]]) != want["digests"][i]]
+    assert not bad, f"{len(bad)} clouds differ from the reference, first {bad[:10]}"
+
+
 def test_packed_input_with_empty_clouds(dt):
     rng = np.random.default_rng(21)
     sizes = [0, 1200, 0, 0, 5, 4000, 0]

@@ -184,8 +184,8 @@ def test_back_to_back_calls_reuse_the_workspace(dt):
 
 def test_wide_keys_take_the_tail_passes(dt):
     """Cell keys wider than 32 bits (x and y spread over 2e6 with eps 0.01:
-    about 29 bits each) need more than four radix passes, run by the walking
-    tail launch with a grid barrier between passes; single and batched calls
+    about 29 bits each) need more than four radix passes, run by the tail
+    launch whose workgroups claim tiles by ticket; single and batched calls
     against the oracle."""
     rng = np.random.default_rng(41)
     far = rng.uniform(-1e6, 1e6, (40_000, 2))
@@ -275,6 +275,46 @@ def test_stage_intermediates_match_oracle(dt):
     assert np.array_equal(has_core_gpu.astype(bool), status_ref != 0)
     assert np.array_equal(cell_cluster[order], ref["cell_cluster"])
     assert int(labels.max()) + 1 == ref["n_clusters"]
+
+
+def test_reference_debug_binding_matches_oracle(dt):
+    """_C.dbscan2d_cuda_debug returns the reference's 11 tensors in its order
+    (csrc/cuda/dbscan_cuda.cu:772-800), in the reference CPU path's point and
+    cell order; each equals the oracle's stage on lattice1m (keys at exact-eps
+    lattice ties, ky straddling no sign, 182k cells)."""
+    X, eps, ms = W.lattice1m()
+    ref = oracle.dbscan2d_debug(X, eps, ms)
+    out = [t.cpu().numpy() for t in dt._C.dbscan2d_cuda_debug(torch.from_numpy(X).to(DEV), eps, ms)]
+    assert len(out) == 11
+    labels, keys, perm, xs, ys, core, cstart, ccl, uf, pre, fin = out
+    assert keys.dtype == np.int64 and perm.dtype == np.int32 and core.dtype == np.uint8
+    assert np.array_equal(labels, ref["labels"])
+    sizes = np.diff(ref["cell_start"])
+    key_ref = (ref["cell_kx"].astype(np.int64) << 32) | (ref["cell_ky"].astype(np.int64) & 0xFFFFFFFF)
+    assert np.array_equal(keys, np.repeat(key_ref, sizes))
+    assert (np.diff(keys) >= 0).all()  # signed ascending
+    assert np.array_equal(perm, ref["perm"])
+    assert np.array_equal(xs, X[ref["perm"], 0]) and np.array_equal(ys, X[ref["perm"], 1])
+    assert np.array_equal(core, ref["is_core"])
+    assert np.array_equal(cstart, ref["cell_start"])
+    assert np.array_equal(ccl, ref["cell_cluster"])
+    assert np.array_equal(uf, ref["cell_root"])
+    core_lab = np.repeat(ref["cell_cluster"], sizes)
+    assert np.array_equal(pre, np.where(ref["is_core"] == 1, core_lab, -1))
+    assert np.array_equal(fin, ref["labels"][ref["perm"]])
+
+
+def test_reference_debug_binding_negative_keys(dt):
+    """Keys of cells with kx < 0 and ky < 0 (signed order across zero)."""
+    X = W.G(40_000, 10, (-15, 15), 0.7, 0.1, (-18, 18), 5)
+    ref = oracle.dbscan2d_debug(X, 0.5, 5)
+    out = [t.cpu().numpy() for t in dt._C.dbscan2d_cuda_debug(torch.from_numpy(X).to(DEV), 0.5, 5)]
+    sizes = np.diff(ref["cell_start"])
+    key_ref = (ref["cell_kx"].astype(np.int64) << 32) | (ref["cell_ky"].astype(np.int64) & 0xFFFFFFFF)
+    assert (ref["cell_kx"] < 0).any() and (ref["cell_ky"] < 0).any()
+    assert np.array_equal(out[1], np.repeat(key_ref, sizes))
+    assert np.array_equal(out[2], ref["perm"]) and np.array_equal(out[6], ref["cell_start"])
+    assert np.array_equal(out[10], ref["labels"][ref["perm"]])
 
 
 def test_host_tensor_in_host_tensor_out(dt):

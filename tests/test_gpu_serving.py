@@ -3,6 +3,10 @@ labels of a plain dbscan2d call on the same cloud alone (and the oracle's),
 whatever else shares its batch; batches overlap on the device; admission
 respects the slot size and the resident cap."""
 
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -105,3 +109,59 @@ def test_admission_limits(srv):
     assert server.batches == [3, 3, 1]
     assert all((r.labels == 0).all() for r in reqs)
     assert server.footprint_bytes(1000) <= server.arena_bytes
+
+
+def _digest(labels):
+    return hashlib.sha256(np.ascontiguousarray(labels, dtype=np.int32).tobytes()).hexdigest()[:16]
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("rate", W.SERVE_PINNED_RATES)
+def test_pinned_serve_stream_matches_reference(srv, golden_dir, rate):
+    """The first 300 requests of the open-loop Poisson stream at `rate` req/s
+    (N mixture 70% log-U[1k,50k], 25% log-U[50k,1M], 5% log-U[1M,10M], each
+    request its own cloud), served open-loop through the continuous-batching
+    server: every result equals the compiled reference's labels
+    (tests/golden/streams.json)."""
+    with open(os.path.join(golden_dir, "streams.json")) as f:
+        want = json.load(f)["serve"][str(rate)]
+    arrivals, ns, eps, ms = W.serve_stream(rate, W.SERVE_PINNED_COUNT)
+    reqs = []
+    for i in range(W.SERVE_PINNED_COUNT):
+        n_i, e_i, m_i, _ = want["requests"][i]
+        assert (n_i, m_i) == (int(ns[i]), int(ms[i])) and np.float32(e_i) == eps[i]
+        X = torch.from_numpy(W.serve_request_cloud(rate, i, n_i)).pin_memory()
+        reqs.append(srv.Request(i, X, float(eps[i]), int(ms[i])))
+        if i % 100 == 99:
+            print(f"[gpu serve] rate {rate}: {i + 1} requests generated", flush=True)
+    server = srv.Server(slots=3, slot_points=32 << 20, slot_clouds=4096)
+    print(f"[gpu serve] rate {rate}: serving {len(reqs)} requests, {int(ns.sum())} points",
+          flush=True)
+    srv.run_open_loop(server, reqs, arrivals)
+    assert len(server.completed) == len(reqs) and not server.failed
+    bad = [r.rid for r in reqs if _digest(r.labels.numpy()) != want["requests"][r.rid][3]]
+    assert not bad, f"{len(bad)} requests differ from the reference, first {bad[:10]}"
+    s = srv.latency_summary(reqs)
+    print(f"[gpu serve] rate {rate}: {s}", flush=True)
+    assert s["requests"] == len(reqs)
+
+
+def test_device_inputs_written_asynchronously(srv):
+    """Device-resident requests whose points are still being produced on the
+    caller's stream when they are submitted: the server's slot stream waits
+    for them, so every result equals the oracle on the finished points."""
+    server = srv.Server(slots=2, slot_points=1 << 20, slot_clouds=16, cap_bytes=4e9)
+    big = torch.randn(4096, 4096, device=DEV)
+    reqs = []
+    for i in range(8):
+        for _ in range(4):  # keep the current stream busy ahead of the points
+            big = big @ big
+            big = big / big.norm()
+        X = (torch.rand(20_000, 2, device=DEV, generator=None) * 60.0).contiguous()
+        reqs.append(srv.Request(i, X, 0.5, 5))
+        server.submit(reqs[-1])
+        server.step()
+    server.drain()
+    for r in reqs:
+        assert r.error is None
+        assert np.array_equal(r.labels.numpy(), oracle.dbscan2d(r.X.cpu().numpy(), 0.5, 5)), r.rid

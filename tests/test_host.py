@@ -64,7 +64,7 @@ def test_host_tensor_without_device_raises():
 
 def _declared_functions():
     text = open(HEADER).read()
-    return re.findall(r"^(?:int|const char\*)\s+(dbscan2d_\w+)\s*\(", text, re.M)
+    return re.findall(r"^(?:int|void|const char\*)\s+(dbscan2d_\w+)\s*\(", text, re.M)
 
 
 def test_library_exports_every_declared_symbol():
@@ -225,3 +225,45 @@ def test_synthetic_tuning_cloud_is_seeded():
     a, b = synthetic_cloud(5000, 3), synthetic_cloud(5000, 3)
     assert a.shape == (5000, 2) and a.dtype == torch.float32 and torch.equal(a, b)
     assert not torch.equal(a, synthetic_cloud(5000, 4))
+
+
+def test_device_error_bits_map_to_runtime_errors(lib):
+    """Every device-side error bit the header declares has a name, and a
+    non-zero word raises RuntimeError naming each of its bits (host logic
+    only: no device call)."""
+    text = open(HEADER).read()
+    bits = {name: int(v) for name, v in re.findall(r"#define DBSCAN_HIP_DEV_(\w+) (\d+)", text)}
+    assert set(bits) == {"SORT_WAIT", "SCAN_WAIT", "UF_LOOP", "RANGE"}
+    lib.dbscan2d_hip_device_error_name.restype = ctypes.c_char_p
+    lib.dbscan2d_hip_device_error_name.argtypes = [ctypes.c_int32]
+    names = {}
+    for key, bit in bits.items():
+        name = lib.dbscan2d_hip_device_error_name(bit)
+        assert name, key
+        names[bit] = name.decode()
+        assert names[bit] in dbscan_torch._C.device_error_message(bit)
+        with pytest.raises(RuntimeError, match=re.escape(names[bit])):
+            dbscan_torch._C.raise_for_device_error(bit)
+    assert lib.dbscan2d_hip_device_error_name(16) is None
+    word = sum(bits.values())
+    msg = dbscan_torch._C.device_error_message(word)
+    assert all(n in msg for n in names.values()) and "0xf" in msg and "INT32_MIN" in msg
+    dbscan_torch._C.raise_for_device_error(0)  # a clean word raises nothing
+    lib.dbscan2d_hip_status_string.argtypes = [ctypes.c_int]
+    assert b"device-side check failed" in lib.dbscan2d_hip_status_string(9)
+
+
+def test_error_readback_argument_checks(lib):
+    P = ctypes.c_void_p
+    lib.dbscan2d_hip_error_async.argtypes = [P, P, P]
+    lib.dbscan2d_hip_check.argtypes = [P, P, P]
+    word = ctypes.c_int32(0)
+    assert lib.dbscan2d_hip_error_async(None, ctypes.addressof(word), None) == 4
+    assert lib.dbscan2d_hip_error_async(0x1000, None, None) == 4
+    assert lib.dbscan2d_hip_error_async(0x1001, ctypes.addressof(word), None) == 6
+    assert lib.dbscan2d_hip_check(None, None, None) == 4
+    assert lib.dbscan2d_hip_check(0x1010, None, ctypes.addressof(word)) == 6
+
+
+def test_check_device_errors_without_calls():
+    dbscan_torch.check_device_errors()  # nothing in flight: returns
