@@ -48,6 +48,13 @@
 
 #include "dbscan_hip.h"
 
+// Workspace of dbscan2d_hip_noalloc: one array per GPU, placed by the code
+// object loader, held until the process ends, shared by every call on that
+// GPU.  In host code its name is only a handle, never a device address: the
+// address comes from hipGetSymbolAddress, once per device (scratch_address).
+static_assert(DBSCAN_HIP_NOALLOC_SCRATCH_BYTES % 256 == 0, "whole 256-byte blocks");
+static __device__ __align__(256) unsigned char scratch[DBSCAN_HIP_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 constexpr int32_t kNoise = -1;
@@ -3632,6 +3639,30 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   return DBSCAN_HIP_OK;
 }
 
+std::atomic<void*> g_scratch_addr[kMaxDevices];
+
+// The current device's scratch address, looked up on its first use.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return DBSCAN_HIP_ERR_N;
+  void* p = g_scratch_addr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_last_hip_error = static_cast<int>(e != hipSuccess ? e : hipErrorInvalidSymbol);
+      return DBSCAN_HIP_ERR_HIP;
+    }
+    g_scratch_addr[dev].store(p, std::memory_order_release);
+  }
+  *out = p;
+  return DBSCAN_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3647,6 +3678,21 @@ int dbscan2d_hip(const float* X, int64_t n, double eps, int64_t min_samples, int
                  void* workspace, size_t workspace_bytes, hipStream_t stream) {
   return run(X, n, eps, min_samples, labels, workspace, workspace_bytes, stream, nullptr,
              nullptr, nullptr);
+}
+
+int dbscan2d_hip_noalloc(const float* X, int64_t n, double eps, int64_t min_samples,
+                         int32_t* labels, hipStream_t stream) {
+  if (n < 0 || n > INT32_MAX) return DBSCAN_HIP_ERR_N;
+  if (!(eps > 0.0)) return DBSCAN_HIP_ERR_EPS;
+  if (min_samples < 1) return DBSCAN_HIP_ERR_MIN_SAMPLES;
+  if (n == 0) return DBSCAN_HIP_OK;
+  if (n > DBSCAN_HIP_NOALLOC_MAX_POINTS || make_layout(n).total > DBSCAN_HIP_NOALLOC_SCRATCH_BYTES)
+    return DBSCAN_HIP_ERR_TOO_LARGE;
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc != DBSCAN_HIP_OK) return rc;
+  return run(X, n, eps, min_samples, labels, ws, DBSCAN_HIP_NOALLOC_SCRATCH_BYTES, stream,
+             nullptr, nullptr, nullptr);
 }
 
 int dbscan2d_hip_default_config(int64_t n, dbscan_hip_config* cfg) {
@@ -3761,6 +3807,9 @@ const char* dbscan2d_hip_status_string(int status) {
     case DBSCAN_HIP_ERR_CONFIG: return "invalid launch configuration";
     case DBSCAN_HIP_ERR_DEVICE:
       return "a device-side check failed; every label of the call is INT32_MIN";
+    case DBSCAN_HIP_ERR_TOO_LARGE:
+      return "n exceeds DBSCAN_HIP_NOALLOC_MAX_POINTS, the limit of the allocation-free entry "
+             "point (use dbscan2d_hip with a workspace)";
     default: return "unknown status";
   }
 }

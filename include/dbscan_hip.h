@@ -34,7 +34,8 @@ enum {
   DBSCAN_HIP_ERR_ALIGN = 6,          /* X, labels or workspace misaligned          */
   DBSCAN_HIP_ERR_HIP = 7,            /* a HIP runtime call failed (see last error) */
   DBSCAN_HIP_ERR_CONFIG = 8,         /* invalid dbscan_hip_config                  */
-  DBSCAN_HIP_ERR_DEVICE = 9          /* a device-side check failed (dbscan2d_hip_check) */
+  DBSCAN_HIP_ERR_DEVICE = 9,         /* a device-side check failed (dbscan2d_hip_check) */
+  DBSCAN_HIP_ERR_TOO_LARGE = 10      /* n above DBSCAN_HIP_NOALLOC_MAX_POINTS        */
 };
 
 /*
@@ -221,6 +222,27 @@ typedef struct {
 } dbscan_hip_layout;
 
 int dbscan2d_hip_layout(int64_t n, dbscan_hip_layout* layout);
+
+/*
+ * Allocation-free variant of dbscan2d_hip, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * takes no workspace: the pipeline runs in a file-scope device array of
+ * DBSCAN_HIP_NOALLOC_SCRATCH_BYTES bytes that each GPU the library runs on
+ * holds until the process ends; its device address is looked up once per
+ * device on the first call there.  The call allocates nothing and enqueues
+ * only kernels on `stream`.  n may be at most DBSCAN_HIP_NOALLOC_MAX_POINTS
+ * (DBSCAN_HIP_ERR_TOO_LARGE above it).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (on different streams, or from
+ * different threads without ordering) race and must not be made; calls
+ * ordered on one stream are safe.  Each call clears what it needs, so the
+ * previous call's contents do not matter.  A failed device-side check leaves
+ * every label INT32_MIN.  Labels equal dbscan2d_hip's.
+ */
+#define DBSCAN_HIP_NOALLOC_SCRATCH_BYTES (160u << 20)
+#define DBSCAN_HIP_NOALLOC_MAX_POINTS (1 << 20)
+int dbscan2d_hip_noalloc(const float* X, int64_t n, double eps, int64_t min_samples,
+                         int32_t* labels, hipStream_t stream);
 
 /* Human-readable text for a status code. */
 const char* dbscan2d_hip_status_string(int status);
