@@ -286,9 +286,14 @@ def batched_leg(dbscan_torch, steps: int, warmup: int) -> dict:
     }
 
 
-def serving_leg(dbscan_torch, rates, count: int, pool: int = 192) -> dict:
+SERVER_CONFIG = dict(slots=3, slot_points=8 << 20, slot_clouds=4096, big_slots=3,
+                     big_points=24 << 20, big_threshold=1 << 20)
+
+
+def serving_leg(dbscan_torch, rates, count: int, pool: int = 384) -> dict:
     """Open-loop Poisson request streams served by the continuous-batching
-    server (pooled arena, 3 slots of 32M points): p50 / p99 latency and
+    server (pooled arena: 3 slots of 8M points for requests below 1M points,
+    3 slots of 24M points for the larger ones): p50 / p99 latency and
     clouds/s per rate, against serial plain dbscan2d calls on the same
     requests.  Request i of a stream draws eps_i and min_samples_i from the
     stream and its points from a pool of `pool` pre-generated pinned clouds
@@ -306,7 +311,7 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 192) -> dict:
               for j, k in enumerate(pool_ns)]
     print(f"[bench] serving: pool of {pool} clouds, {int(pool_ns.sum())} points "
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    server = Server(slots=3, slot_points=32 << 20, slot_clouds=4096)
+    server = Server(**SERVER_CONFIG)
 
     def make(rate):
         # about 4 s of stream per rate (at least 200 requests, at most `count`)
@@ -321,7 +326,9 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 192) -> dict:
     out = {"workload": "serve: open-loop Poisson, N mixture 70% log-U[1k,50k], 25% "
                        "log-U[50k,1M], 5% log-U[1M,10M], eps U[0.2,1.0], min_samples {5,10,20}",
            "max_requests_per_rate": count, "pool_clouds": pool,
+           "requests_per_rate": "min(max_requests_per_rate, max(200, 4 s x rate))",
            "arena_gb": round(server.arena_bytes / 1e9, 2), "slots": len(server.slots),
+           "server": SERVER_CONFIG,
            "rates": {}}
     for rate in rates:
         arr, reqs = make(rate)
@@ -371,7 +378,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--serving-only", action="store_true",
                     help="run only the serving leg (prints its JSON)")
-    ap.add_argument("--serve-rates", default="100,500,1000,2000,5000",
+    ap.add_argument("--serve-rates", default="100,250,500,1000,2000,5000",
                     help="comma-separated request rates (req/s) of the serving leg")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="most requests per rate in the serving leg (about 4 s of stream)")

@@ -12,9 +12,15 @@ arrive while others run are taken into the next batch as soon as a slot
 frees, and their copies and kernels overlap the batches in flight.  A
 request's labels are exactly what ``dbscan2d`` gives its cloud alone.
 
+Admission is size-aware when ``big_slots`` > 0: requests of at least
+``big_threshold`` points wait in their own queue and go only to the big
+slots (each of ``big_points`` points), so the host-to-device copies and
+kernels of multi-million-point requests never sit in front of small ones; a
+big slot with no big request waiting takes small requests.
+
 Typical use::
 
-    server = Server(slots=3, slot_points=32 << 20)
+    server = Server(slots=3, slot_points=8 << 20, big_slots=3, big_points=24 << 20)
     server.submit(Request(0, X_host, eps=0.5, min_samples=10))
     server.drain()          # or call server.step() from your own loop
     labels = server.completed[0].labels
@@ -73,8 +79,9 @@ class _Slot:
     receives its slice of it, so retiring a batch copies nothing on the host
     (a request's labels keep that batch's buffer alive)."""
 
-    def __init__(self, arena: torch.Tensor, offset: int, points: int, ws_bytes: int,
-                 device: torch.device):
+    def __init__(self, arena: torch.Tensor, offset: int, points: int, clouds: int,
+                 ws_bytes: int, device: torch.device, big: bool = False):
+        self.points, self.clouds, self.big = points, clouds, big
         x_bytes, l_bytes = _align(points * 8), _align(points * 4)
         self.X = arena.narrow(0, offset, points * 8).view(torch.float32).view(points, 2)
         self.labels = arena.narrow(0, offset + x_bytes, points * 4).view(torch.int32)
@@ -97,32 +104,48 @@ class Server:
 
     Args:
         device: the ROCm device.
-        slots: batches that may be on the device at once.
-        slot_points: points one batch holds (the largest admissible request).
+        slots: batches of small requests that may be on the device at once.
+        slot_points: points one such batch holds (without big slots: the
+            largest admissible request).
         slot_clouds: requests one batch holds.
-        cap_bytes: the resident HBM budget; the arena (slots x slot bytes)
-            must fit in it.
+        cap_bytes: the resident HBM budget; the arena (all slots) must fit.
         clock: time source for admission and completion stamps.
+        big_slots: slots reserved for requests of at least ``big_threshold``
+            points (0: one queue, every slot takes every request).
+        big_points: points one big slot holds (the largest admissible
+            request when there are big slots).
+        big_threshold: the size from which a request is big.
     """
 
     def __init__(self, device="cuda", slots: int = 3, slot_points: int = 32 << 20,
                  slot_clouds: int = 4096, cap_bytes: float = 200e9,
-                 clock: Callable[[], float] = time.perf_counter):
-        if slots < 1 or slot_points < 1 or slot_clouds < 1:
+                 clock: Callable[[], float] = time.perf_counter, big_slots: int = 0,
+                 big_points: Optional[int] = None, big_threshold: int = 1 << 20):
+        if slots < 1 or slot_points < 1 or slot_clouds < 1 or big_slots < 0:
             raise ValueError("Server: slots, slot_points and slot_clouds must be >= 1")
         self.device = torch.device(device)
         self.slot_points, self.slot_clouds, self.clock = slot_points, slot_clouds, clock
+        self.big_points = (big_points or slot_points) if big_slots else 0
+        self.big_threshold = big_threshold if big_slots else None
+        if big_slots and self.big_points < big_threshold:
+            raise ValueError("Server: big_points must be at least big_threshold")
         ws = _C.dbscan2d_batched_workspace_bytes(slot_points, slot_clouds)
         slot_bytes = _Slot.nbytes(slot_points, ws)
-        self.arena_bytes = slots * slot_bytes
+        big_ws = _C.dbscan2d_batched_workspace_bytes(self.big_points, slot_clouds) if big_slots else 0
+        big_bytes = _Slot.nbytes(self.big_points, big_ws) if big_slots else 0
+        self.arena_bytes = slots * slot_bytes + big_slots * big_bytes
         if self.arena_bytes > cap_bytes:
             raise ValueError(f"Server: arena of {self.arena_bytes / 1e9:.1f} GB exceeds the "
                              f"{cap_bytes / 1e9:.1f} GB resident cap")
         self.arena = torch.empty(self.arena_bytes, dtype=torch.uint8, device=self.device)
-        self.slots = [_Slot(self.arena, i * slot_bytes, slot_points, ws, self.device)
+        self.slots = [_Slot(self.arena, i * slot_bytes, slot_points, slot_clouds, ws, self.device)
                       for i in range(slots)]
+        self.slots += [_Slot(self.arena, slots * slot_bytes + i * big_bytes, self.big_points,
+                             slot_clouds, big_ws, self.device, big=True)
+                       for i in range(big_slots)]
         self.bytes_per_point = slot_bytes / slot_points
-        self.queue: Deque[Request] = collections.deque()
+        self.queue: Deque[Request] = collections.deque()      # small (or every) request
+        self.big_queue: Deque[Request] = collections.deque()  # big requests
         self.free: Deque[_Slot] = collections.deque(self.slots)
         self.inflight: Deque[_Slot] = collections.deque()
         self.completed: Dict[int, Request] = {}
@@ -144,27 +167,41 @@ class Server:
             raise RuntimeError("eps must be > 0")
         if req.min_samples < 1:
             raise RuntimeError("min_samples must be >= 1")
-        if req.n > self.slot_points:
+        big = self.big_threshold is not None and (req.n >= self.big_threshold or
+                                                  req.n > self.slot_points)
+        limit = self.big_points if big else self.slot_points
+        if req.n > limit:
             raise ValueError(f"Server: request {req.rid} has {req.n} points; a slot holds "
-                             f"{self.slot_points}")
+                             f"{limit}")
         if req.n == 0:
             req.labels = torch.empty(0, dtype=torch.int32)
             req.admitted = req.done = self.clock()
             self.completed[req.rid] = req
             return
-        self.queue.append(req)
+        if big:
+            self.big_queue.append(req)
+        else:
+            self.queue.append(req)
 
     def busy(self) -> bool:
-        return bool(self.queue) or bool(self.inflight)
+        return bool(self.queue) or bool(self.big_queue) or bool(self.inflight)
 
-    def _launch(self, slot: _Slot) -> None:
+    def _source(self, slot: _Slot) -> Optional[Deque[Request]]:
+        """The queue a free slot takes requests from, or None."""
+        if slot.big and self.big_queue:
+            return self.big_queue
+        return self.queue if self.queue else None
+
+    def _launch(self, slot: _Slot, queue: Deque[Request]) -> None:
         batch, offs, pts = [], [0], 0
-        while (self.queue and len(batch) < self.slot_clouds
-               and pts + self.queue[0].n <= self.slot_points):
-            r = self.queue.popleft()
+        while (queue and len(batch) < slot.clouds and pts + queue[0].n <= slot.points):
+            r = queue.popleft()
             batch.append(r)
             pts += r.n
             offs.append(pts)
+        if not batch:  # the head request does not fit this slot
+            self.free.append(slot)
+            return
         now = self.clock()
         for r in batch:
             r.admitted = now
@@ -212,9 +249,12 @@ class Server:
                 self.inflight.remove(slot)
                 self._retire(slot)
                 progressed = True
-        while self.queue and self.free:
-            self._launch(self.free.popleft())
-            progressed = True
+        for slot in list(self.free):
+            queue = self._source(slot)
+            if queue is not None:
+                self.free.remove(slot)
+                self._launch(slot, queue)
+                progressed = True
         return progressed
 
     def drain(self) -> None:

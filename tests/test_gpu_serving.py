@@ -134,7 +134,10 @@ def test_pinned_serve_stream_matches_reference(srv, golden_dir, rate):
         reqs.append(srv.Request(i, X, float(eps[i]), int(ms[i])))
         if i % 100 == 99:
             print(f"[gpu serve] rate {rate}: {i + 1} requests generated", flush=True)
-    server = srv.Server(slots=3, slot_points=32 << 20, slot_clouds=4096)
+    # the benchmark's size-aware configuration: requests of 1M points and more
+    # in their own slots
+    server = srv.Server(slots=3, slot_points=8 << 20, slot_clouds=4096, big_slots=3,
+                        big_points=24 << 20, big_threshold=1 << 20)
     print(f"[gpu serve] rate {rate}: serving {len(reqs)} requests, {int(ns.sum())} points",
           flush=True)
     srv.run_open_loop(server, reqs, arrivals)
@@ -165,3 +168,41 @@ def test_device_inputs_written_asynchronously(srv):
     for r in reqs:
         assert r.error is None
         assert np.array_equal(r.labels.numpy(), oracle.dbscan2d(r.X.cpu().numpy(), 0.5, 5)), r.rid
+
+
+def test_size_aware_admission(srv):
+    """Requests of at least big_threshold points go only to the big slots;
+    small requests submitted after them are launched at once on the small
+    slots, and a big slot with no big request waiting takes small ones."""
+    server = srv.Server(slots=2, slot_points=1 << 18, slot_clouds=8, big_slots=1,
+                        big_points=1 << 20, big_threshold=100_000, cap_bytes=8e9)
+    big = [srv.Request(i, torch.from_numpy(W.cloud_at_density(400_000, 8100 + i)).pin_memory(),
+                       0.5, 10) for i in range(3)]
+    small = [srv.Request(10 + i, torch.from_numpy(W.cloud_at_density(5_000, 8200 + i)).pin_memory(),
+                         0.4, 5) for i in range(6)]
+    for r in big + small:
+        server.submit(r)
+    assert len(server.big_queue) == 3 and len(server.queue) == 6
+    server.step()
+    launched = {r.rid for s in server.inflight for r in s.batch}
+    # the big slot took the first two big requests (a third does not fit its
+    # 1M points); the small slots took the small requests, not waiting behind
+    assert {0, 1} & launched and all(r.rid in launched for r in small)
+    assert all(not s.big or all(r.n >= 100_000 for r in s.batch) for s in server.inflight)
+    assert all(s.big or all(r.n < 100_000 for r in s.batch) for s in server.inflight)
+    server.drain()
+    for r in big + small:
+        assert r.error is None
+        assert np.array_equal(r.labels.numpy(), oracle.dbscan2d(r.X.numpy(), r.eps, r.min_samples))
+    with pytest.raises(ValueError, match="a slot holds"):
+        server.submit(srv.Request(99, torch.zeros((2 << 20, 2)), 0.5, 5))
+    # with nothing big waiting, the big slot serves small requests too
+    more = [srv.Request(20 + i, torch.from_numpy(W.cloud_at_density(5_000, 8300 + i)).pin_memory(),
+                        0.4, 5) for i in range(24)]
+    for r in more:
+        server.submit(r)
+    server.step()
+    assert any(s.big for s in server.inflight)
+    server.drain()
+    for r in more:
+        assert np.array_equal(r.labels.numpy(), oracle.dbscan2d(r.X.numpy(), r.eps, r.min_samples))
