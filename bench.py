@@ -353,10 +353,20 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 384) -> dict:
     plain = [dbscan_torch.dbscan2d(r.X.cuda(non_blocking=True), r.eps, r.min_samples).cpu()
              for r in reqs[:k]]
     serial = time.perf_counter() - t0
-    server_reqs = run_open_loop(server, reqs[:k], [0.0] * k)  # all at once: peak throughput
-    burst = max(r.done for r in server_reqs) - min(r.arrival for r in server_reqs)
+    # all at once: peak throughput.  One untimed burst first (its label
+    # buffers of new sizes are pinned by torch's host allocator, which a
+    # running server has already done), then the median of three.
+    bursts = []
+    for rep in range(4):
+        burst_reqs = [Request(r.rid, r.X, r.eps, r.min_samples) for r in reqs[:k]]
+        server_reqs = run_open_loop(server, burst_reqs, [0.0] * k)
+        if rep:
+            bursts.append(max(r.done for r in server_reqs) - min(r.arrival for r in server_reqs))
+    burst = sorted(bursts)[1]
     out["serial_plain_clouds_per_s"] = round(k / serial, 1)
     out["burst_served_clouds_per_s"] = round(k / burst, 1)
+    out["burst_ms"] = [round(b * 1e3, 2) for b in bursts]
+    out["burst_points"] = int(sum(r.n for r in reqs[:k]))
     out["labels_identical_to_plain"] = all(torch.equal(p, r.labels) for p, r in
                                            zip(plain, server_reqs))
     return out

@@ -184,11 +184,27 @@ const dbscan_hip_config* config_for(int64_t n) {
   return g_tuning.empty() ? nullptr : &g_tuning.back().second;
 }
 
-dbscan_hip_config to_config(const std::vector<int64_t>& v) {
-  TORCH_CHECK(v.size() == 4,
-              "config must be (sort_items, radix_bits, cell_blocks, pair_blocks)");
+// (sort_items, radix_bits, cell_blocks, pair_blocks[, small_max]); without
+// small_max, the library's default for that size.
+dbscan_hip_config to_config(const std::vector<int64_t>& v, int64_t n = 0) {
+  TORCH_CHECK(v.size() == 4 || v.size() == 5,
+              "config must be (sort_items, radix_bits, cell_blocks, pair_blocks[, small_max])");
+  dbscan_hip_config d;
+  check_status(dbscan2d_hip_default_config(n, &d));
   return dbscan_hip_config{static_cast<int32_t>(v[0]), static_cast<int32_t>(v[1]),
-                           static_cast<int32_t>(v[2]), static_cast<int32_t>(v[3])};
+                           static_cast<int32_t>(v[2]), static_cast<int32_t>(v[3]),
+                           v.size() == 5 ? static_cast<int32_t>(v[4]) : d.small_max};
+}
+
+// The configuration a call on n points uses, with the one-workgroup path
+// switched off (the debug variants read the pipeline's intermediates).
+dbscan_hip_config pipeline_config(int64_t n) {
+  dbscan_hip_config c;
+  const dbscan_hip_config* t = config_for(n);
+  if (t) c = *t;
+  else check_status(dbscan2d_hip_default_config(n, &c));
+  c.small_max = 0;
+  return c;
 }
 
 struct DeviceRun {
@@ -266,16 +282,17 @@ torch::Tensor dbscan2d_configured(torch::Tensor X, double eps, int64_t min_sampl
                                   std::vector<int64_t> config) {
   check_input(X, eps, min_samples);
   if (X.size(0) == 0) return torch::empty({0}, X.options().dtype(torch::kInt32));
-  const dbscan_hip_config cfg = to_config(config);
+  const dbscan_hip_config cfg = to_config(config, X.size(0));
   return finish(X, run_device(X, eps, min_samples, nullptr, &cfg, check_for(X, false)).labels);
 }
 
 void set_tuning_table(std::vector<std::vector<int64_t>> rows) {
   std::vector<std::pair<int64_t, dbscan_hip_config>> t;
   for (const auto& r : rows) {
-    TORCH_CHECK(r.size() == 5, "tuning rows are (max_n, sort_items, radix_bits, cell_blocks, "
-                               "pair_blocks)");
-    t.emplace_back(r[0], to_config({r[1], r[2], r[3], r[4]}));
+    TORCH_CHECK(r.size() == 5 || r.size() == 6,
+                "tuning rows are (max_n, sort_items, radix_bits, cell_blocks, pair_blocks"
+                "[, small_max])");
+    t.emplace_back(r[0], to_config(std::vector<int64_t>(r.begin() + 1, r.end()), r[0]));
   }
   std::sort(t.begin(), t.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
   g_tuning = std::move(t);
@@ -285,14 +302,14 @@ std::vector<std::vector<int64_t>> get_tuning_table() {
   std::vector<std::vector<int64_t>> out;
   for (const auto& e : g_tuning)
     out.push_back({e.first, e.second.sort_items, e.second.radix_bits, e.second.cell_blocks,
-                   e.second.pair_blocks});
+                   e.second.pair_blocks, e.second.small_max});
   return out;
 }
 
 std::vector<int64_t> default_config(int64_t n) {
   dbscan_hip_config c;
   check_status(dbscan2d_hip_default_config(n, &c));
-  return {c.sort_items, c.radix_bits, c.cell_blocks, c.pair_blocks};
+  return {c.sort_items, c.radix_bits, c.cell_blocks, c.pair_blocks, c.small_max};
 }
 
 // csrc/extension.cpp:52-63: labels plus GPU milliseconds per reference stage.
@@ -328,7 +345,8 @@ std::pair<torch::Tensor, std::map<std::string, float>> dbscan2d_profile(torch::T
 std::vector<torch::Tensor> dbscan2d_hip_debug(torch::Tensor X, double eps, int64_t min_samples) {
   check_input(X, eps, min_samples);
   TORCH_CHECK(X.size(0) > 0, "debug variant requires n > 0");
-  DeviceRun r = run_device(X, eps, min_samples, nullptr, nullptr, Check::kNone);
+  const dbscan_hip_config cfg = pipeline_config(X.size(0));
+  DeviceRun r = run_device(X, eps, min_samples, nullptr, &cfg, Check::kNone);
   const int64_t n = r.Xd.size(0);
   dbscan_hip_layout L;
   check_status(dbscan2d_hip_layout(n, &L));
@@ -378,7 +396,8 @@ std::vector<torch::Tensor> dbscan2d_cuda_debug(torch::Tensor X, double eps, int6
   TORCH_CHECK(X.size(0) > 0, "debug variant requires n > 0");
   TORCH_CHECK(static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(min_samples))) >= 1,
               "debug variant: min_samples narrows below 1 (every label is -1, no stage runs)");
-  DeviceRun r = run_device(X, eps, min_samples, nullptr, nullptr, Check::kNone);
+  const dbscan_hip_config cfg = pipeline_config(X.size(0));
+  DeviceRun r = run_device(X, eps, min_samples, nullptr, &cfg, Check::kNone);
   const int64_t n = r.Xd.size(0);
   dbscan_hip_layout L;
   check_status(dbscan2d_hip_layout(n, &L));
@@ -636,7 +655,7 @@ dbscan2d_benchmark(torch::Tensor X, double eps, int64_t min_samples, int64_t ste
   dbscan_hip_config cfg{};
   const dbscan_hip_config* cp = config_for(n);
   if (config) {
-    cfg = to_config(*config);
+    cfg = to_config(*config, n);
     cp = &cfg;
   }
   auto r = time_kernels(steps, [&](hipEvent_t* ev) {
@@ -668,7 +687,7 @@ dbscan2d_batched_benchmark(torch::Tensor X, std::vector<int64_t> offsets, std::v
   dbscan_hip_config cfg{};
   const dbscan_hip_config* cp = config_for(n);
   if (config) {
-    cfg = to_config(*config);
+    cfg = to_config(*config, n);
     cp = &cfg;
   }
   auto r = time_kernels(steps, [&](hipEvent_t* ev) {

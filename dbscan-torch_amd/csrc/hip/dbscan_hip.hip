@@ -2994,6 +2994,585 @@ __global__ __launch_bounds__(64) void k_copy_error(const Params* __restrict__ pr
 }
 
 // ---------------------------------------------------------------------------
+// Small clouds: one workgroup clusters a whole cloud in LDS
+// ---------------------------------------------------------------------------
+//
+// A cloud of at most kSmallMax points runs the reference's stages
+// (dbscan_grid.hpp:159-454) inside one workgroup, its working set in LDS,
+// with barriers between the stages instead of the pipeline's ~20 launches
+// (which are most of a small call's time):
+//   1. points into LDS, cell-coordinate bounds (block min/max);
+//   2. stable LSD radix sort of 16-bit point indices by the compressed key
+//      x' << by | y' (8-bit digits; wave64 ballot ranking into per-wave
+//      counters, as in the pipeline's radix pass), keys recomputed from the
+//      points each pass; the points are then gathered into sorted order;
+//   3. cell runs by a block scan: point -> cell, cell starts and keys;
+//   4. per cell: its 5x5 window as five (first cell, count) column ranges by
+//      binary search over the cell keys, and its reference-order index (a
+//      rotation inside its column when ky straddles 0, as in k_ngh);
+//   5. core marking (dbscan_grid.hpp:267-307): a full cell wholesale, else the
+//      own cell wholesale plus the distance test over the other 24 cells,
+//      after the exact window-count prune;
+//   6. union-find over reference indices in LDS (dbscan_grid.hpp:351-387): a
+//      core point tests each later core window cell of another root for an
+//      eps-close core point; CAS hooks the larger root onto the smaller;
+//   7. cluster ids (dbscan_grid.hpp:392-398): roots ranked in reference order
+//      by a block scan;
+//   8. labels (dbscan_grid.hpp:403-454): a core point takes its cell's id, any
+//      other the first strict minimum of (d2, window offset) over the cores
+//      within eps; written to input order.
+// Window columns are searched as k_ngh searches them: a neighbouring column
+// outside the cloud's x' range holds no cell, and rows do not wrap (a cell
+// coordinate is at most 2^31 - 128, so kx + 2 never wraps past INT32_MAX).
+// Distances use the pipeline's test, fl(fl(ex*ex) + fl(ey*ey)) <= eps_sq.
+constexpr int kSmallMax = DBSCAN_HIP_SMALL_MAX;
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallWaves = kSmallThreads / 64;
+constexpr int kSmallItems = kSmallMax / kSmallThreads;  // points per thread
+constexpr int kSmallDigitBits = 8;
+constexpr int kSmallDigits = 1 << kSmallDigitBits;
+constexpr uint16_t kSmallNone = 0xFFFFu;  // cell without a cluster
+static_assert(kSmallMax % kSmallThreads == 0, "whole items per thread");
+static_assert(kSmallMax < 0xFFFF, "16-bit point and cell indices, 0xFFFF free");
+static_assert(kSmallDigits <= kSmallThreads, "a thread per digit");
+
+struct SmallLds {
+  float x[kSmallMax], y[kSmallMax];  // input order, then sorted order
+  uint16_t idx[2][kSmallMax];        // sort buffers; then sorted -> input, point -> cell
+  uint8_t core[kSmallMax];           // by sorted point
+  uint8_t has_core[kSmallMax];       // by cell
+  uint16_t cstart[kSmallMax + 8];    // cell runs, C + 1 entries
+  uint16_t cref[kSmallMax];          // reference-order index of each cell
+  union {
+    uint32_t wcnt[kSmallWaves][kSmallDigits];  // sort: per-wave digit counters
+    uint64_t ckey[kSmallMax];                  // cells: compressed keys, ascending
+    struct {
+      uint32_t uf[kSmallMax];    // union-find parent, by reference index
+      uint16_t rank[kSmallMax];  // root flags, then root ranks, by reference index
+      uint16_t ccl[kSmallMax];   // cluster id by cell, or kSmallNone
+    } u;
+  } r;
+  uint16_t wbase[kSmallMax][5];  // window column k (dx = k - 2): first cell
+  uint16_t wlen[kSmallMax];      // its cell count (at most 5), 3 bits per column
+  uint32_t dstart[kSmallDigits];
+  uint32_t scan[kSmallWaves];
+  int32_t red[kSmallWaves][4];
+  int32_t err;
+  int32_t ticket;
+};
+static_assert(sizeof(SmallLds) <= 160 * 1024, "one workgroup's LDS on gfx950");
+
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Root of x in the LDS forest, halving the path (parents only move to
+// smaller indices, so every walk ends).  Bounded: a longer walk can only come
+// from a corrupted forest (kErrUfLoop, stop there).
+__device__ __forceinline__ uint32_t small_find(uint32_t* uf, uint32_t x, int32_t* err) {
+  uint32_t p = lds_ld(uf + x);
+  for (int it = 0; p != x; ++it) {
+    if (it > kSmallMax) {
+      atomicOr(err, static_cast<int32_t>(kErrUfLoop));
+      break;
+    }
+    const uint32_t gp = lds_ld(uf + p);
+    if (gp != p) lds_st(uf + x, gp);
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+// A CAS fails only when another thread has just hooked that root, which
+// happens at most C - 1 times, so 2 * kSmallMax attempts always suffice.
+__device__ __forceinline__ void small_unite(uint32_t* uf, uint32_t a, uint32_t b, int32_t* err) {
+  for (int it = 0; it < 2 * kSmallMax; ++it) {
+    a = small_find(uf, a, err);
+    b = small_find(uf, b, err);
+    if (a == b) return;
+    const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+    if (atomicCAS(uf + hi, hi, lo) == hi) return;
+  }
+  atomicOr(err, static_cast<int32_t>(kErrUfLoop));
+}
+
+// One cloud (ci: points [begin, end), parameters as a single call computes
+// them), by the whole workgroup; returns after a closing barrier.
+__device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, const float2* __restrict__ X,
+                            int32_t* __restrict__ labels) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // single-value stores are made by all of wave 0 (a wave-uniform branch)
+  const bool wave0 = __builtin_amdgcn_readfirstlane(t) == 0;
+  const int n = ci.end - ci.begin;
+  if (n <= 0) return;
+  if (n > kSmallMax) {  // the host routes only small clouds here
+    if (wave0) atomicOr(&s.err, static_cast<int32_t>(kErrRange));
+    return;
+  }
+  const float inv_r = ci.inv_r, eps_sq = ci.eps_sq;
+  const int32_t ms = ci.ms;
+
+  // 1. points and cell bounds
+  int32_t b4[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  for (int i = t; i < n; i += kSmallThreads) {
+    const float2 p = X[ci.begin + i];
+    s.x[i] = p.x;
+    s.y[i] = p.y;
+    const int32_t kx = cell_coord(p.x * inv_r), ky = cell_coord(p.y * inv_r);
+    b4[0] = min(b4[0], kx);
+    b4[1] = max(b4[1], kx);
+    b4[2] = min(b4[2], ky);
+    b4[3] = max(b4[3], ky);
+  }
+  block_minmax4<kSmallWaves>(b4, s.red);  // ends with a barrier
+  const uint32_t xmin = static_cast<uint32_t>(b4[0]), ymin = static_cast<uint32_t>(b4[2]);
+  const uint32_t xspan = static_cast<uint32_t>(b4[1]) - xmin;  // x' <= xspan
+  const uint32_t yspan = static_cast<uint32_t>(b4[3]) - ymin;  // y' <= yspan
+  const uint32_t by = bit_width(yspan);
+  const int passes = (static_cast<int>(bit_width(xspan) + by) + kSmallDigitBits - 1) /
+                     kSmallDigitBits;
+  const uint64_t ymask = (uint64_t{1} << by) - 1u;  // by <= 32
+  auto key_of = [&](float x, float y) -> uint64_t {
+    const uint32_t xr = static_cast<uint32_t>(cell_coord(x * inv_r)) - xmin;
+    const uint32_t yr = static_cast<uint32_t>(cell_coord(y * inv_r)) - ymin;
+    return (static_cast<uint64_t>(xr) << by) | yr;
+  };
+
+  // 2. radix sort of point indices.  Item k of lane l in wave w sits at
+  // position w * 64 * kSmallItems + k * 64 + l: ranking item by item keeps
+  // equal digits in position order (stable).
+  for (int i = t; i < n; i += kSmallThreads) s.idx[0][i] = static_cast<uint16_t>(i);
+  int src = 0;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  for (int pass = 0; pass < passes; ++pass) {
+    const uint32_t shift = static_cast<uint32_t>(pass * kSmallDigitBits);
+    for (int e = t; e < kSmallWaves * kSmallDigits; e += kSmallThreads)
+      (&s.r.wcnt[0][0])[e] = 0u;
+    __syncthreads();
+    uint32_t id[kSmallItems], dg[kSmallItems], rk[kSmallItems];
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int p = wave * 64 * kSmallItems + k * 64 + lane;
+      const bool valid = p < n;
+      id[k] = valid ? s.idx[src][p] : 0u;
+      const uint64_t key = valid ? key_of(s.x[id[k]], s.y[id[k]]) : 0ull;
+      dg[k] = static_cast<uint32_t>(key >> shift) & (kSmallDigits - 1u);
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < kSmallDigitBits; ++b) {
+        const bool bit = (dg[k] >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
+      uint32_t old = 0;
+      if (valid) old = s.r.wcnt[wave][dg[k]];
+      if (valid && below == 0)
+        s.r.wcnt[wave][dg[k]] = old + static_cast<uint32_t>(__popcll(peers));
+      rk[k] = old + below;
+    }
+    __syncthreads();
+    // digit t: exclusive offsets of the waves, then of the digits
+    uint32_t tot = 0;
+    if (t < kSmallDigits) {
+      for (int w = 0; w < kSmallWaves; ++w) {
+        const uint32_t c = s.r.wcnt[w][t];
+        s.r.wcnt[w][t] = tot;
+        tot += c;
+      }
+    }
+    uint32_t all;
+    const uint32_t d0 = block_excl_scan<kSmallThreads>(tot, all, s.scan);
+    if (t < kSmallDigits) s.dstart[t] = d0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int p = wave * 64 * kSmallItems + k * 64 + lane;
+      if (p < n) {
+        const uint32_t q = s.dstart[dg[k]] + s.r.wcnt[wave][dg[k]] + rk[k];
+        if (q < static_cast<uint32_t>(n)) s.idx[src ^ 1][q] = static_cast<uint16_t>(id[k]);
+        else atomicOr(&s.err, static_cast<int32_t>(kErrRange));
+      }
+    }
+    src ^= 1;
+    __syncthreads();
+  }
+  const uint16_t* __restrict__ perm = s.idx[src];  // sorted -> input
+  uint16_t* __restrict__ pcell = s.idx[src ^ 1];   // sorted point -> cell
+  {
+    float gx[kSmallItems], gy[kSmallItems];
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int p = t + k * kSmallThreads;
+      if (p < n) {
+        const int i = perm[p];
+        gx[k] = s.x[i];
+        gy[k] = s.y[i];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int p = t + k * kSmallThreads;
+      if (p < n) {
+        s.x[p] = gx[k];
+        s.y[p] = gy[k];
+      }
+    }
+    __syncthreads();
+  }
+
+  // 3. cell runs (thread t: positions [t * kSmallItems, (t + 1) * kSmallItems))
+  int C;
+  {
+    const int p0 = t * kSmallItems;
+    uint64_t kk[kSmallItems];
+    uint64_t prev = (p0 > 0 && p0 - 1 < n) ? key_of(s.x[p0 - 1], s.y[p0 - 1]) : 0ull;
+    uint32_t flags = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int p = p0 + k;
+      kk[k] = 0;
+      if (p < n) {
+        kk[k] = key_of(s.x[p], s.y[p]);
+        if (p == 0 || kk[k] != prev) {
+          flags |= 1u << k;
+          ++cnt;
+        }
+        prev = kk[k];
+      }
+    }
+    uint32_t total;
+    uint32_t c = block_excl_scan<kSmallThreads>(cnt, total, s.scan);
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int p = p0 + k;
+      if (p < n) {
+        if ((flags >> k) & 1u) {
+          s.cstart[c] = static_cast<uint16_t>(p);
+          s.r.ckey[c] = kk[k];
+          ++c;
+        }
+        pcell[p] = static_cast<uint16_t>(c - 1);
+      }
+    }
+    C = static_cast<int>(total);
+    if (wave0) s.cstart[C] = static_cast<uint16_t>(n);
+  }
+  __syncthreads();
+
+  // 4. windows and reference indices
+  const bool straddle = b4[2] < 0 && b4[3] >= 0;
+  const uint32_t yr0 = 0u - ymin;  // y' of ky == 0 (when straddling)
+  const uint64_t* __restrict__ ckey = s.r.ckey;
+  auto lower_bound = [&](uint64_t v) {
+    int lo = 0, len = C;
+    while (len > 0) {
+      const int half = len >> 1;
+      if (ckey[lo + half] < v) {
+        lo += half + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+    return lo;
+  };
+  for (int c = t; c < C; c += kSmallThreads) {
+    const uint64_t key = ckey[c];
+    const uint32_t xr = static_cast<uint32_t>(key >> by);
+    const uint32_t yr = static_cast<uint32_t>(key & ymask);
+    const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
+    const uint32_t yhi = static_cast<uint32_t>(
+        min(static_cast<uint64_t>(yr) + 2u, static_cast<uint64_t>(yspan)));
+    // the five lower bounds of (x' + dx, ylo) in lock-step
+    uint64_t lo_key[5], hi_key[5];
+    int base[5], len[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int64_t cx = static_cast<int64_t>(xr) + (k - 2);
+      const bool live = cx >= 0 && cx <= static_cast<int64_t>(xspan);
+      lo_key[k] = (static_cast<uint64_t>(cx) << by) | ylo;
+      hi_key[k] = (static_cast<uint64_t>(cx) << by) | yhi;
+      base[k] = 0;
+      len[k] = live ? C : 0;
+    }
+    for (int it = 0; it < 16; ++it) {  // C <= 4096: at most 13 halvings
+      bool more = false;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        if (len[k] > 0) {
+          const int half = len[k] >> 1;
+          if (ckey[base[k] + half] < lo_key[k]) {
+            base[k] += half + 1;
+            len[k] -= half + 1;
+          } else {
+            len[k] = half;
+          }
+          more |= len[k] > 0;
+        }
+      }
+      if (!more) break;
+    }
+    uint32_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int64_t cx = static_cast<int64_t>(xr) + (k - 2);
+      const bool live = cx >= 0 && cx <= static_cast<int64_t>(xspan);
+      int cnt = 0;
+      if (live) {
+        // the column's cells at y' in [ylo, yhi]: at most 5 (one per y'),
+        // keys ascending, so a prefix from base
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+          cnt += (cnt == q && base[k] + q < C && ckey[base[k] + q] <= hi_key[k]) ? 1 : 0;
+      }
+      s.wbase[c][k] = static_cast<uint16_t>(live ? base[k] : 0);
+      packed |= static_cast<uint32_t>(cnt) << (3 * k);
+    }
+    s.wlen[c] = static_cast<uint16_t>(packed);
+    int r = c;
+    if (straddle) {
+      const uint64_t col = static_cast<uint64_t>(xr) << by;
+      const int own_lo = lower_bound(col);
+      const int own_hi = xr < xspan ? lower_bound(static_cast<uint64_t>(xr + 1u) << by) : C;
+      const int split = lower_bound(col | yr0);
+      r = (yr >= yr0) ? own_lo + (c - split) : c + (own_hi - split);
+    }
+    if (r < 0 || r >= C) {
+      atomicOr(&s.err, static_cast<int32_t>(kErrRange));
+      r = c;
+    }
+    s.cref[c] = static_cast<uint16_t>(r);
+    s.has_core[c] = 0;
+  }
+  __syncthreads();  // the cell keys are dead from here on (their space becomes uf)
+
+  // window cells of column k of cell c: [wbase, wbase + count)
+  auto win = [&](int c, int k, int& b, int& e) {
+    b = s.wbase[c][k];
+    e = b + ((s.wlen[c] >> (3 * k)) & 7);
+  };
+
+  // 5. core marking
+  uint32_t* __restrict__ uf = s.r.u.uf;
+  for (int r = t; r < C; r += kSmallThreads) uf[r] = static_cast<uint32_t>(r);
+  for (int p = t; p < n; p += kSmallThreads) {
+    const int c = pcell[p];
+    const int size = s.cstart[c + 1] - s.cstart[c];
+    bool core = size >= ms;  // dbscan_grid.hpp:271-279
+    if (!core) {
+      int pop = 0;  // the window's points: fewer than ms, no core
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        int b, e;
+        win(c, k, b, e);
+        pop += s.cstart[e] - s.cstart[b];
+      }
+      if (pop >= ms) {
+        const float xi = s.x[p], yi = s.y[p];
+        int count = size;  // own cell wholesale (dbscan_grid.hpp:293-296)
+        for (int k = 0; k < 5 && count < ms; ++k) {
+          int b, e;
+          win(c, k, b, e);
+          for (int nc = b; nc < e && count < ms; ++nc) {
+            if (nc == c) continue;
+            const int j1 = s.cstart[nc + 1];
+            for (int j = s.cstart[nc]; j < j1; ++j) {
+              const float ex = s.x[j] - xi, ey = s.y[j] - yi;
+              if (ex * ex + ey * ey <= eps_sq) ++count;
+            }
+          }
+        }
+        core = count >= ms;
+      }
+    }
+    s.core[p] = core ? 1 : 0;
+    if (core) s.has_core[c] = 1;  // equal stores only
+  }
+  __syncthreads();
+
+  // 6. union-find over reference indices
+  for (int p = t; p < n; p += kSmallThreads) {
+    if (!s.core[p]) continue;
+    const int c = pcell[p];
+    const uint32_t rc = s.cref[c];
+    const float xi = s.x[p], yi = s.y[p];
+    for (int k = 0; k < 5; ++k) {
+      int b, e;
+      win(c, k, b, e);
+      for (int nc = b; nc < e; ++nc) {
+        if (nc <= c || !s.has_core[nc]) continue;  // each cell pair once
+        const uint32_t rn = s.cref[nc];
+        if (small_find(uf, rc, &s.err) == small_find(uf, rn, &s.err)) continue;
+        const int j1 = s.cstart[nc + 1];
+        for (int j = s.cstart[nc]; j < j1; ++j) {
+          if (!s.core[j]) continue;
+          const float ex = s.x[j] - xi, ey = s.y[j] - yi;
+          if (ex * ex + ey * ey <= eps_sq) {
+            small_unite(uf, rc, rn, &s.err);
+            break;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // 7. cluster ids: every reference index to its root, roots ranked in
+  // reference order
+  {
+    uint32_t root[kSmallItems];
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int r = t + k * kSmallThreads;
+      root[k] = r < C ? small_find(uf, static_cast<uint32_t>(r), &s.err) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      const int r = t + k * kSmallThreads;
+      if (r < C) uf[r] = root[k];
+    }
+    __syncthreads();
+  }
+  for (int c = t; c < C; c += kSmallThreads) {
+    const uint32_t r = s.cref[c];
+    s.r.u.rank[r] = (s.has_core[c] && uf[r] == r) ? 1 : 0;
+  }
+  __syncthreads();
+  {
+    const int r0 = t * kSmallItems;
+    uint32_t f[kSmallItems], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      f[k] = r0 + k < C ? s.r.u.rank[r0 + k] : 0u;
+      sum += f[k];
+    }
+    uint32_t roots;
+    uint32_t off = block_excl_scan<kSmallThreads>(sum, roots, s.scan);
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+      if (r0 + k < C) s.r.u.rank[r0 + k] = static_cast<uint16_t>(off);
+      off += f[k];
+    }
+  }
+  __syncthreads();
+  for (int c = t; c < C; c += kSmallThreads)
+    s.r.u.ccl[c] = s.has_core[c] ? s.r.u.rank[uf[s.cref[c]]] : kSmallNone;
+  __syncthreads();
+
+  // 8. labels, in input order
+  const float best_init = ci.best_init;
+  for (int p = t; p < n; p += kSmallThreads) {
+    const int c = pcell[p];
+    int32_t lab;
+    if (s.core[p]) {
+      lab = s.r.u.ccl[c];
+    } else {
+      // dbscan_grid.hpp:419-445: window offsets in order (dx-major, then dy,
+      // which is the column's cell order); a strictly smaller d2 wins
+      lab = kNoise;
+      float best = best_init;
+      const float xi = s.x[p], yi = s.y[p];
+      for (int k = 0; k < 5; ++k) {
+        int b, e;
+        win(c, k, b, e);
+        for (int nc = b; nc < e; ++nc) {
+          if (!s.has_core[nc]) continue;
+          const int32_t cl = s.r.u.ccl[nc];
+          const int j1 = s.cstart[nc + 1];
+          for (int j = s.cstart[nc]; j < j1; ++j) {
+            if (!s.core[j]) continue;
+            const float ex = s.x[j] - xi, ey = s.y[j] - yi;
+            const float d2 = ex * ex + ey * ey;
+            if (d2 <= eps_sq && d2 < best) {
+              best = d2;
+              lab = cl;
+            }
+          }
+        }
+      }
+    }
+    const int i = perm[p];
+    if (i < n) labels[ci.begin + i] = lab;
+    else atomicOr(&s.err, static_cast<int32_t>(kErrRange));
+  }
+  __syncthreads();
+}
+
+// Clears the batch-mode words of k_small (ticket, finished count, error word).
+__global__ __launch_bounds__(64) void k_small_init(Params* __restrict__ prm, int32_t error0) {
+  if (threadIdx.x == 0) {
+    prm->error = error0;
+    prm->tile_counter[0] = 0u;
+    prm->tile_counter[1] = 0u;
+  }
+}
+
+// Clusters num_clouds small clouds, one workgroup at a time per cloud
+// (clouds == nullptr: the single cloud `one`).  One workgroup walks the
+// clouds in order and owns the error word (error0 is where it starts).  A
+// larger grid is persistent: its workgroups take clouds from an atomic
+// ticket (cleared by k_small_init, which also set the error word), OR their
+// errors into the word, and the last one to finish reads it.  Either way a
+// call whose word ends non-zero has every one of its `total` labels
+// overwritten with kPoison, and the word goes to error_out (if not null).
+__global__ __launch_bounds__(kSmallThreads) void k_small(
+    const float2* __restrict__ X, int32_t* __restrict__ labels,
+    const CloudInfo* __restrict__ clouds, int32_t num_clouds, CloudInfo one,
+    Params* __restrict__ prm, int32_t error0, int32_t* __restrict__ error_out, int64_t total) {
+  __shared__ SmallLds s;
+  // branches on these are uniform across a wave (see k_sort_tail)
+  const bool wave0 = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x)) == 0;
+  const uint32_t one_inc = threadIdx.x == 0 ? 1u : 0u;
+  if (wave0) s.err = 0;
+  __syncthreads();
+  for (int32_t k = -1;;) {
+    if (gridDim.x == 1) {
+      ++k;
+    } else {
+      if (wave0) {
+        const uint32_t old = atomicAdd(&prm->tile_counter[0], one_inc);
+        s.ticket = __builtin_amdgcn_readlane(static_cast<int>(old), 0);
+      }
+      __syncthreads();
+      k = __builtin_amdgcn_readfirstlane(s.ticket);
+      __syncthreads();
+    }
+    if (k >= num_clouds) break;
+    small_cloud(s, clouds ? clouds[k] : one, X, labels);
+  }
+  __syncthreads();
+  const int32_t local = __builtin_amdgcn_readfirstlane(s.err);
+  int32_t bits;
+  if (gridDim.x == 1) {
+    bits = local | error0;
+    if (wave0) prm->error = bits;
+  } else {
+    if (local && wave0) atomicOr(&prm->error, local);
+    __threadfence();
+    __syncthreads();
+    if (wave0) {
+      const uint32_t old = atomicAdd(&prm->tile_counter[1], one_inc);
+      s.ticket = __builtin_amdgcn_readlane(static_cast<int>(old), 0);
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(s.ticket) != static_cast<int32_t>(gridDim.x) - 1) return;
+    __threadfence();
+    bits = __builtin_amdgcn_readfirstlane(ld_acquire(&prm->error));
+  }
+  if (bits) {
+    for (int64_t i = threadIdx.x; i < total; i += kSmallThreads) labels[i] = kPoison;
+  }
+  if (error_out && wave0) error_out[0] = bits;
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 
@@ -3167,18 +3746,24 @@ constexpr int kStageSlots[DBSCAN_HIP_NUM_STAGES][2] = {
 // (Widening radix_bits changes nothing unless it saves a pass: the digit
 // width is the key width over the pass count.  At 10M points, 9 bits sort
 // 25-27-bit keys in 3 passes of about 94 us instead of 4 of about 74 us.)
+// Clouds of at most kSmallDefault points take the one-workgroup path: the
+// size below which it beat the pipeline (single calls of constant-density
+// clouds on one MI355X: 58 against 114 us at 100 points, even at 1,000).
+constexpr int kSmallDefault = 1000;
+
 dbscan_hip_config default_config(int64_t n) {
-  if (n <= 31622) return dbscan_hip_config{8, 8, 2048, 16384};
-  if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 4096};
-  if (n <= 3162277) return dbscan_hip_config{8, 7, 8192, 8192};
-  if (n <= (int64_t{1} << 24)) return dbscan_hip_config{12, 9, 4096, 8192};
-  return dbscan_hip_config{12, 9, 8192, 8192};  // mostly batches of many clouds
+  if (n <= 31622) return dbscan_hip_config{8, 8, 2048, 16384, kSmallDefault};
+  if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 4096, kSmallDefault};
+  if (n <= 3162277) return dbscan_hip_config{8, 7, 8192, 8192, kSmallDefault};
+  if (n <= (int64_t{1} << 24)) return dbscan_hip_config{12, 9, 4096, 8192, kSmallDefault};
+  return dbscan_hip_config{12, 9, 8192, 8192, kSmallDefault};  // mostly batches of many clouds
 }
 
 bool config_ok(const dbscan_hip_config& c) {
   return (c.sort_items == 8 || c.sort_items == 12 || c.sort_items == 16) && c.radix_bits >= 4 &&
          c.radix_bits <= kRadixBitsMax && c.cell_blocks >= 64 && c.cell_blocks <= 65536 &&
-         c.pair_blocks >= 64 && c.pair_blocks <= 65536;
+         c.pair_blocks >= 64 && c.pair_blocks <= 65536 && c.small_max >= 0 &&
+         c.small_max <= kSmallMax;
 }
 
 // Passes from kTailPass on are needed only for keys wider than 4 digits of
@@ -3297,8 +3882,27 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
 
   void* ws = workspace;
   Params* prm = at<Params>(ws, L.params);
-  uint64_t* sort_status = at<uint64_t>(ws, L.sort_status);
   const float2* X2 = reinterpret_cast<const float2*>(X);
+
+  // A small cloud: one workgroup, one launch (the profile and per-kernel
+  // timing variants report the pipeline's stages, so they keep it).
+  if (n <= cfg.small_max && !stage_ms && !kev) {
+    CloudInfo one;
+    std::memset(&one, 0, sizeof(one));
+    one.begin = 0;
+    one.end = static_cast<int32_t>(n);
+    one.inv_r = inv_r;
+    one.eps_sq = eps_sq;
+    one.best_init = best_init;
+    one.ms = ms;
+    k_small<<<1, kSmallThreads, 0, stream>>>(X2, labels, nullptr, 1, one, prm,
+                                             take_injected_error(), error_out, n);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    release();
+    return DBSCAN_HIP_OK;
+  }
+
+  uint64_t* sort_status = at<uint64_t>(ws, L.sort_status);
   float* xs_a = at<float>(ws, L.xs_a);
   float* ys_a = at<float>(ws, L.ys_a);
   uint32_t* idx_a = at<uint32_t>(ws, L.idx_a);
@@ -3490,6 +4094,35 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const Layout L = make_layout(total, num_clouds);
   if (workspace_bytes < L.total) return DBSCAN_HIP_ERR_WORKSPACE;
 
+  auto fail = [&](hipError_t e) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  };
+  hipError_t e;
+  void* ws = workspace;
+  const float2* X2 = reinterpret_cast<const float2*>(X);
+  Params* prm = at<Params>(ws, L.params);
+
+  // Every cloud small: a persistent grid of one-workgroup clusterings that
+  // takes the clouds from a device ticket (the per-kernel timing variant
+  // keeps the pipeline).
+  bool all_small = kev == nullptr && cfg.small_max > 0;
+  for (int32_t i = 0; i < B && all_small; ++i)
+    all_small = info[i].end - info[i].begin <= cfg.small_max;
+  if (all_small) {
+    CloudInfo* clouds = at<CloudInfo>(ws, L.clouds);
+    if ((e = hipMemcpyAsync(clouds, info.data(), sizeof(CloudInfo) * B, hipMemcpyHostToDevice,
+                            stream)) != hipSuccess)
+      return fail(e);
+    const int grid = static_cast<int>(std::min<int64_t>(B, cu_count()));
+    const int32_t error0 = take_injected_error();
+    if (grid > 1) k_small_init<<<1, 64, 0, stream>>>(prm, error0);
+    k_small<<<grid, kSmallThreads, 0, stream>>>(X2, labels, clouds, B, CloudInfo{}, prm,
+                                                grid > 1 ? 0 : error0, error_out, total);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    return DBSCAN_HIP_OK;
+  }
+
   // Cloud records and tile tables, staged on the host and copied in one go
   // (the copy is ordered before the kernels on `stream`).
   std::vector<char> staging(L.seg_end - L.clouds, 0);
@@ -3507,27 +4140,19 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
     }
   }
 
-  auto fail = [&](hipError_t e) {
-    g_last_hip_error = static_cast<int>(e);
-    return DBSCAN_HIP_ERR_HIP;
-  };
   auto before = [&](int k) -> hipError_t {
     return kev ? hipEventRecord(kev[k], stream) : hipSuccess;
   };
-  hipError_t e;
-  void* ws = workspace;
   if ((e = hipMemcpyAsync(at<char>(ws, L.clouds), staging.data(), staging.size(),
                           hipMemcpyHostToDevice, stream)) != hipSuccess)
     return fail(e);
 
   const int64_t n = total;
-  Params* prm = at<Params>(ws, L.params);
   uint64_t* sort_status = at<uint64_t>(ws, L.sort_status);
   uint32_t* cloud_hist = at<uint32_t>(ws, L.cloud_hist);
   CloudInfo* clouds = at<CloudInfo>(ws, L.clouds);
   const SegTile* seg_sort = at<SegTile>(ws, L.seg_sort);
   const SegTile* seg_scan = at<SegTile>(ws, L.seg_scan);
-  const float2* X2 = reinterpret_cast<const float2*>(X);
   float* xs_a = at<float>(ws, L.xs_a);
   float* ys_a = at<float>(ws, L.ys_a);
   uint32_t* idx_a = at<uint32_t>(ws, L.idx_a);

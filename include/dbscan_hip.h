@@ -58,7 +58,10 @@ enum {
 /*
  * Launch configuration (no counterpart in the reference, whose launch shapes
  * are fixed in csrc/cuda/dbscan_cuda.cu).  Every valid configuration gives
- * the same labels; they differ in speed only.  The entry points without a
+ * the same labels; they differ in speed only.  A call (or a batch whose
+ * clouds all have) at most small_max points runs as one workgroup per cloud
+ * with its working set in LDS, instead of the multi-kernel pipeline; the
+ * profile, timed and debug entry points always run the pipeline.  The entry points without a
  * configuration argument (or given NULL) use the built-in table for the
  * input size (dbscan2d_hip_default_config), which the project's autotuner
  * (dbscan_torch.autotune) produces and can override per process.
@@ -68,7 +71,13 @@ typedef struct {
   int32_t radix_bits;   /* widest radix digit used when it saves no pass: 4..9   */
   int32_t cell_blocks;  /* workgroups (of 256) of the per-cell kernels: 64..65536 */
   int32_t pair_blocks;  /* workgroups of the cell-pair union kernels: 64..65536  */
+  int32_t small_max;    /* clouds of at most this many points are clustered by
+                           one workgroup in LDS (one launch); 0: never;
+                           0..DBSCAN_HIP_SMALL_MAX                           */
 } dbscan_hip_config;
+
+/* Largest cloud the one-workgroup path takes (its LDS holds the whole cloud). */
+#define DBSCAN_HIP_SMALL_MAX 4096
 
 /* The built-in configuration for n points. */
 int dbscan2d_hip_default_config(int64_t n, dbscan_hip_config* cfg);
