@@ -3032,6 +3032,19 @@ constexpr int kSmallItems = kSmallMax / kSmallThreads;  // points per thread
 constexpr int kSmallDigitBits = 8;
 constexpr int kSmallDigits = 1 << kSmallDigitBits;
 constexpr uint16_t kSmallNone = 0xFFFFu;  // cell without a cluster
+// keys of at most this many bits are sorted by one counting pass
+constexpr int kSmallCountBits = 14;
+// clouds spanning fewer cell columns get a column table
+constexpr int kSmallCols = 2048;
+// a point whose window holds more points than this is done by a wave
+constexpr int kSmallLaneScan = 64;
+// lanes that test one cell pair together (union), one queued point (core)
+constexpr int kSmallPairGroup = 8;
+constexpr int kSmallCoreGroup = 16;
+// clouds of at most this many cells get cell boxes for the union prunes
+constexpr int kSmallBoxes = 1024;
+// tests a lane makes on a cell pair before it queues the pair for a group
+constexpr int kSmallLaneTests = 32;
 static_assert(kSmallMax % kSmallThreads == 0, "whole items per thread");
 static_assert(kSmallMax < 0xFFFF, "16-bit point and cell indices, 0xFFFF free");
 static_assert(kSmallDigits <= kSmallThreads, "a thread per digit");
@@ -3054,12 +3067,20 @@ struct SmallLds {
   } r;
   uint16_t wbase[kSmallMax][5];  // window column k (dx = k - 2): first cell
   uint16_t wlen[kSmallMax];      // its cell count (at most 5), 3 bits per column
+  uint16_t col[kSmallCols + 8];  // first cell of column x' (narrow clouds)
   uint32_t dstart[kSmallDigits];
   uint32_t scan[kSmallWaves];
   int32_t red[kSmallWaves][4];
   int32_t err;
   int32_t ticket;
+  int32_t qcore;   // points queued for a wave (core marking, then border)
 };
+static_assert(kSmallBoxes * sizeof(uint4) <= 2 * sizeof(uint16_t) * kSmallMax,
+              "cell boxes in the rank and id space");
+static_assert((1 << kSmallCountBits) / 2 * sizeof(uint32_t) <= sizeof(SmallLds::r),
+              "16-bit counters, two to a word, in the cell-key space");
+static_assert((1 << kSmallCountBits) / 2 % kSmallThreads == 0, "whole counter words per thread");
+static_assert(kSmallMax < (1 << 16), "offsets and counts fit 16 bits");
 static_assert(sizeof(SmallLds) <= 160 * 1024, "one workgroup's LDS on gfx950");
 
 __device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
@@ -3067,6 +3088,22 @@ __device__ __forceinline__ uint32_t lds_ld(const uint32_t* p) {
 }
 __device__ __forceinline__ void lds_st(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// This lane's group of G lanes (G divides 64) in a wave ballot.
+template <int G>
+__device__ __forceinline__ uint64_t group_bits(uint64_t ballot) {
+  if constexpr (G == 64) {
+    return ballot;
+  } else {
+    const int base = static_cast<int>(threadIdx.x & 63) & ~(G - 1);
+    return (ballot >> base) & ((uint64_t{1} << G) - 1u);
+  }
+}
+
+// The o-th window column, nearest first (dx = 0, -1, 1, -2, 2).
+__device__ __forceinline__ int near_column(int o) {
+  return o == 0 ? 2 : ((o & 1) ? 2 - (o + 1) / 2 : 2 + o / 2);
 }
 
 // Root of x in the LDS forest, halving the path (parents only move to
@@ -3102,8 +3139,9 @@ __device__ __forceinline__ void small_unite(uint32_t* uf, uint32_t a, uint32_t b
 
 // One cloud (ci: points [begin, end), parameters as a single call computes
 // them), by the whole workgroup; returns after a closing barrier.
-__device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, const float2* __restrict__ X,
-                            int32_t* __restrict__ labels) {
+__device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci,
+                                            const float2* __restrict__ X,
+                                            int32_t* __restrict__ labels) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   // single-value stores are made by all of wave 0 (a wave-uniform branch)
   const bool wave0 = __builtin_amdgcn_readfirstlane(t) == 0;
@@ -3133,8 +3171,7 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
   const uint32_t xspan = static_cast<uint32_t>(b4[1]) - xmin;  // x' <= xspan
   const uint32_t yspan = static_cast<uint32_t>(b4[3]) - ymin;  // y' <= yspan
   const uint32_t by = bit_width(yspan);
-  const int passes = (static_cast<int>(bit_width(xspan) + by) + kSmallDigitBits - 1) /
-                     kSmallDigitBits;
+  const int nbits = static_cast<int>(bit_width(xspan) + by);
   const uint64_t ymask = (uint64_t{1} << by) - 1u;  // by <= 32
   auto key_of = [&](float x, float y) -> uint64_t {
     const uint32_t xr = static_cast<uint32_t>(cell_coord(x * inv_r)) - xmin;
@@ -3142,65 +3179,119 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
     return (static_cast<uint64_t>(xr) << by) | yr;
   };
 
-  // 2. radix sort of point indices.  Item k of lane l in wave w sits at
-  // position w * 64 * kSmallItems + k * 64 + l: ranking item by item keeps
-  // equal digits in position order (stable).
-  for (int i = t; i < n; i += kSmallThreads) s.idx[0][i] = static_cast<uint16_t>(i);
+  // 2. point indices sorted by key.  No label depends on the order of a
+  // cell's points, so a key of at most kSmallCountBits bits takes one
+  // counting pass (LDS atomics); a wider one stable LSD passes of 8-bit digits.
   int src = 0;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
-  for (int pass = 0; pass < passes; ++pass) {
-    const uint32_t shift = static_cast<uint32_t>(pass * kSmallDigitBits);
-    for (int e = t; e < kSmallWaves * kSmallDigits; e += kSmallThreads)
-      (&s.r.wcnt[0][0])[e] = 0u;
+  if (nbits == 0) {  // one cell
+    for (int i = t; i < n; i += kSmallThreads) s.idx[0][i] = static_cast<uint16_t>(i);
+  } else if (nbits <= kSmallCountBits) {
+    // one 16-bit counter per key, two to a word (counts are at most 4,096)
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(&s.r);
+    for (int e = t; e < (1 << kSmallCountBits) / 2; e += kSmallThreads) cnt[e] = 0u;
     __syncthreads();
-    uint32_t id[kSmallItems], dg[kSmallItems], rk[kSmallItems];
+    uint32_t key[kSmallItems], rk[kSmallItems];
 #pragma unroll
     for (int k = 0; k < kSmallItems; ++k) {
-      const int p = wave * 64 * kSmallItems + k * 64 + lane;
-      const bool valid = p < n;
-      id[k] = valid ? s.idx[src][p] : 0u;
-      const uint64_t key = valid ? key_of(s.x[id[k]], s.y[id[k]]) : 0ull;
-      dg[k] = static_cast<uint32_t>(key >> shift) & (kSmallDigits - 1u);
-      uint64_t peers = __ballot(valid);
-#pragma unroll
-      for (int b = 0; b < kSmallDigitBits; ++b) {
-        const bool bit = (dg[k] >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
+      const int i = t + k * kSmallThreads;
+      key[k] = 0;
+      rk[k] = 0;
+      if (i < n) {
+        key[k] = static_cast<uint32_t>(key_of(s.x[i], s.y[i]));
+        const uint32_t sh = (key[k] & 1u) * 16u;
+        rk[k] = (atomicAdd(cnt + (key[k] >> 1), 1u << sh) >> sh) & 0xFFFFu;
       }
-      const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
-      uint32_t old = 0;
-      if (valid) old = s.r.wcnt[wave][dg[k]];
-      if (valid && below == 0)
-        s.r.wcnt[wave][dg[k]] = old + static_cast<uint32_t>(__popcll(peers));
-      rk[k] = old + below;
     }
     __syncthreads();
-    // digit t: exclusive offsets of the waves, then of the digits
-    uint32_t tot = 0;
-    if (t < kSmallDigits) {
-      for (int w = 0; w < kSmallWaves; ++w) {
-        const uint32_t c = s.r.wcnt[w][t];
-        s.r.wcnt[w][t] = tot;
-        tot += c;
+    {
+      constexpr int kWords = (1 << kSmallCountBits) / 2 / kSmallThreads;
+      uint32_t w[kWords], sum = 0;
+#pragma unroll
+      for (int q = 0; q < kWords; ++q) {
+        w[q] = cnt[t * kWords + q];
+        sum += (w[q] & 0xFFFFu) + (w[q] >> 16);
+      }
+      uint32_t all;
+      uint32_t off = block_excl_scan<kSmallThreads>(sum, all, s.scan);
+#pragma unroll
+      for (int q = 0; q < kWords; ++q) {
+        const uint32_t lo = off, hi = off + (w[q] & 0xFFFFu);
+        cnt[t * kWords + q] = lo | (hi << 16);
+        off = hi + (w[q] >> 16);
       }
     }
-    uint32_t all;
-    const uint32_t d0 = block_excl_scan<kSmallThreads>(tot, all, s.scan);
-    if (t < kSmallDigits) s.dstart[t] = d0;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSmallItems; ++k) {
-      const int p = wave * 64 * kSmallItems + k * 64 + lane;
-      if (p < n) {
-        const uint32_t q = s.dstart[dg[k]] + s.r.wcnt[wave][dg[k]] + rk[k];
-        if (q < static_cast<uint32_t>(n)) s.idx[src ^ 1][q] = static_cast<uint16_t>(id[k]);
+      const int i = t + k * kSmallThreads;
+      if (i < n) {
+        const uint32_t sh = (key[k] & 1u) * 16u;
+        const uint32_t q = ((cnt[key[k] >> 1] >> sh) & 0xFFFFu) + rk[k];
+        if (q < static_cast<uint32_t>(n)) s.idx[0][q] = static_cast<uint16_t>(i);
         else atomicOr(&s.err, static_cast<int32_t>(kErrRange));
       }
     }
-    src ^= 1;
-    __syncthreads();
+  } else {
+    // Item k of lane l in wave w sits at position w * 64 * kSmallItems +
+    // k * 64 + l: ranking item by item keeps equal digits in position order.
+    for (int i = t; i < n; i += kSmallThreads) s.idx[0][i] = static_cast<uint16_t>(i);
+    const int passes = (nbits + kSmallDigitBits - 1) / kSmallDigitBits;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (int pass = 0; pass < passes; ++pass) {
+      const uint32_t shift = static_cast<uint32_t>(pass * kSmallDigitBits);
+      for (int e = t; e < kSmallWaves * kSmallDigits; e += kSmallThreads)
+        (&s.r.wcnt[0][0])[e] = 0u;
+      __syncthreads();
+      uint32_t id[kSmallItems], dg[kSmallItems], rk[kSmallItems];
+#pragma unroll
+      for (int k = 0; k < kSmallItems; ++k) {
+        const int p = wave * 64 * kSmallItems + k * 64 + lane;
+        const bool valid = p < n;
+        id[k] = valid ? s.idx[src][p] : 0u;
+        const uint64_t key = valid ? key_of(s.x[id[k]], s.y[id[k]]) : 0ull;
+        dg[k] = static_cast<uint32_t>(key >> shift) & (kSmallDigits - 1u);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kSmallDigitBits; ++b) {
+          const bool bit = (dg[k] >> b) & 1u;
+          const uint64_t bal = __ballot(bit);
+          peers &= bit ? bal : ~bal;
+        }
+        const uint32_t below = static_cast<uint32_t>(__popcll(peers & lt_mask));
+        uint32_t old = 0;
+        if (valid) old = s.r.wcnt[wave][dg[k]];
+        if (valid && below == 0)
+          s.r.wcnt[wave][dg[k]] = old + static_cast<uint32_t>(__popcll(peers));
+        rk[k] = old + below;
+      }
+      __syncthreads();
+      // digit t: exclusive offsets of the waves, then of the digits
+      uint32_t tot = 0;
+      if (t < kSmallDigits) {
+        for (int w = 0; w < kSmallWaves; ++w) {
+          const uint32_t c = s.r.wcnt[w][t];
+          s.r.wcnt[w][t] = tot;
+          tot += c;
+        }
+      }
+      uint32_t all;
+      const uint32_t d0 = block_excl_scan<kSmallThreads>(tot, all, s.scan);
+      if (t < kSmallDigits) s.dstart[t] = d0;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kSmallItems; ++k) {
+        const int p = wave * 64 * kSmallItems + k * 64 + lane;
+        if (p < n) {
+          const uint32_t q = s.dstart[dg[k]] + s.r.wcnt[wave][dg[k]] + rk[k];
+          if (q < static_cast<uint32_t>(n)) s.idx[src ^ 1][q] = static_cast<uint16_t>(id[k]);
+          else atomicOr(&s.err, static_cast<int32_t>(kErrRange));
+        }
+      }
+      src ^= 1;
+      __syncthreads();
+    }
   }
+  __syncthreads();
   const uint16_t* __restrict__ perm = s.idx[src];  // sorted -> input
   uint16_t* __restrict__ pcell = s.idx[src ^ 1];   // sorted point -> cell
   {
@@ -3261,16 +3352,31 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
       }
     }
     C = static_cast<int>(total);
-    if (wave0) s.cstart[C] = static_cast<uint16_t>(n);
+    if (wave0) {
+      s.cstart[C] = static_cast<uint16_t>(n);
+      s.qcore = 0;
+    }
   }
   __syncthreads();
 
-  // 4. windows and reference indices
-  const bool straddle = b4[2] < 0 && b4[3] >= 0;
-  const uint32_t yr0 = 0u - ymin;  // y' of ky == 0 (when straddling)
+  // 4. windows and reference indices.  Narrow clouds first get a column
+  // table (col[x] = first cell of column x), so that every search below runs
+  // inside one column; wider ones search all cells.
   const uint64_t* __restrict__ ckey = s.r.ckey;
-  auto lower_bound = [&](uint64_t v) {
-    int lo = 0, len = C;
+  const bool coltab = xspan < static_cast<uint32_t>(kSmallCols);
+  if (coltab) {
+    for (int c = t; c < C; c += kSmallThreads) {
+      const int xr = static_cast<int>(ckey[c] >> by);
+      const int prev = c > 0 ? static_cast<int>(ckey[c - 1] >> by) : -1;
+      for (int x = prev + 1; x <= xr; ++x) s.col[x] = static_cast<uint16_t>(c);
+      if (c == C - 1)
+        for (int x = xr + 1; x <= static_cast<int>(xspan) + 1; ++x)
+          s.col[x] = static_cast<uint16_t>(C);
+    }
+    __syncthreads();
+  }
+  // first cell of [lo, lo + len) whose key is >= v
+  auto lower_bound = [&](int lo, int len, uint64_t v) {
     while (len > 0) {
       const int half = len >> 1;
       if (ckey[lo + half] < v) {
@@ -3282,6 +3388,18 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
     }
     return lo;
   };
+  // cells of column x (x <= xspan): [lo, hi)
+  auto column = [&](uint32_t x, int& lo, int& hi) {
+    if (coltab) {
+      lo = s.col[x];
+      hi = s.col[x + 1];
+    } else {
+      lo = lower_bound(0, C, static_cast<uint64_t>(x) << by);
+      hi = x < xspan ? lower_bound(lo, C - lo, static_cast<uint64_t>(x + 1u) << by) : C;
+    }
+  };
+  const bool straddle = b4[2] < 0 && b4[3] >= 0;
+  const uint32_t yr0 = 0u - ymin;  // y' of ky == 0 (when straddling)
   for (int c = t; c < C; c += kSmallThreads) {
     const uint64_t key = ckey[c];
     const uint32_t xr = static_cast<uint32_t>(key >> by);
@@ -3289,19 +3407,29 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
     const uint32_t ylo = yr >= 2u ? yr - 2u : 0u;
     const uint32_t yhi = static_cast<uint32_t>(
         min(static_cast<uint64_t>(yr) + 2u, static_cast<uint64_t>(yspan)));
-    // the five lower bounds of (x' + dx, ylo) in lock-step
+    // the five lower bounds of (x' + dx, ylo), in lock-step, each inside its
+    // column when there is a column table
     uint64_t lo_key[5], hi_key[5];
     int base[5], len[5];
+    bool live[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const int64_t cx = static_cast<int64_t>(xr) + (k - 2);
-      const bool live = cx >= 0 && cx <= static_cast<int64_t>(xspan);
+      live[k] = cx >= 0 && cx <= static_cast<int64_t>(xspan);
       lo_key[k] = (static_cast<uint64_t>(cx) << by) | ylo;
       hi_key[k] = (static_cast<uint64_t>(cx) << by) | yhi;
       base[k] = 0;
-      len[k] = live ? C : 0;
+      len[k] = 0;
+      if (live[k]) {
+        if (coltab) {
+          base[k] = s.col[cx];
+          len[k] = s.col[cx + 1] - base[k];
+        } else {
+          len[k] = C;
+        }
+      }
     }
-    for (int it = 0; it < 16; ++it) {  // C <= 4096: at most 13 halvings
+    for (int it = 0; it < 16; ++it) {  // at most 13 halvings
       bool more = false;
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -3321,26 +3449,26 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
     uint32_t packed = 0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      const int64_t cx = static_cast<int64_t>(xr) + (k - 2);
-      const bool live = cx >= 0 && cx <= static_cast<int64_t>(xspan);
-      int cnt = 0;
-      if (live) {
-        // the column's cells at y' in [ylo, yhi]: at most 5 (one per y'),
-        // keys ascending, so a prefix from base
+      // the column's cells at y' in [ylo, yhi]: at most 5 (one per y'),
+      // keys ascending, so a prefix from base (its length: the trailing
+      // ones of the mask)
+      uint32_t m = 0;
 #pragma unroll
-        for (int q = 0; q < 5; ++q)
-          cnt += (cnt == q && base[k] + q < C && ckey[base[k] + q] <= hi_key[k]) ? 1 : 0;
+      for (int q = 0; q < 5; ++q) {
+        const int cq = min(base[k] + q, C - 1);
+        m |= (live[k] && base[k] + q < C && ckey[cq] <= hi_key[k]) ? (1u << q) : 0u;
       }
-      s.wbase[c][k] = static_cast<uint16_t>(live ? base[k] : 0);
+      const int cnt = __builtin_ctz(~m);
+      s.wbase[c][k] = static_cast<uint16_t>(live[k] ? base[k] : 0);
       packed |= static_cast<uint32_t>(cnt) << (3 * k);
     }
     s.wlen[c] = static_cast<uint16_t>(packed);
     int r = c;
     if (straddle) {
-      const uint64_t col = static_cast<uint64_t>(xr) << by;
-      const int own_lo = lower_bound(col);
-      const int own_hi = xr < xspan ? lower_bound(static_cast<uint64_t>(xr + 1u) << by) : C;
-      const int split = lower_bound(col | yr0);
+      int own_lo, own_hi;
+      column(xr, own_lo, own_hi);
+      const int split =
+          lower_bound(own_lo, own_hi - own_lo, (static_cast<uint64_t>(xr) << by) | yr0);
       r = (yr >= yr0) ? own_lo + (c - split) : c + (own_hi - split);
     }
     if (r < 0 || r >= C) {
@@ -3352,42 +3480,52 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
   }
   __syncthreads();  // the cell keys are dead from here on (their space becomes uf)
 
-  // window cells of column k of cell c: [wbase, wbase + count)
-  auto win = [&](int c, int k, int& b, int& e) {
-    b = s.wbase[c][k];
-    e = b + ((s.wlen[c] >> (3 * k)) & 7);
+  // The window of cell c as five ranges of sorted points, one per window
+  // column (dx = k - 2): a column's window cells are consecutive, and so are
+  // their points.  In scan order (dx, then dy, then the point) the ranges
+  // and the points in them ascend.
+  auto window = [&](int c, int (&lo)[5], int (&hi)[5]) {
+    const uint32_t wl = s.wlen[c];
+    int pop = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int b = s.wbase[c][k];
+      lo[k] = s.cstart[b];
+      hi[k] = s.cstart[b + ((wl >> (3 * k)) & 7u)];
+      pop += hi[k] - lo[k];
+    }
+    return pop;
+  };
+  auto dist_ok = [&](int j, float xi, float yi) {
+    const float ex = s.x[j] - xi, ey = s.y[j] - yi;
+    return ex * ex + ey * ey <= eps_sq;
   };
 
-  // 5. core marking
+  // 5. core marking (dbscan_grid.hpp:267-307): a full cell wholesale; else
+  // the own cell wholesale plus the eps-close points of the other 24 cells,
+  // counted by the point's lane when its window is small and by a wave
+  // (64 points a step) otherwise.
   uint32_t* __restrict__ uf = s.r.u.uf;
+  uint16_t* __restrict__ pqueue = s.r.u.rank;  // free until stage 7
   for (int r = t; r < C; r += kSmallThreads) uf[r] = static_cast<uint32_t>(r);
   for (int p = t; p < n; p += kSmallThreads) {
     const int c = pcell[p];
-    const int size = s.cstart[c + 1] - s.cstart[c];
+    const int own_lo = s.cstart[c], own_hi = s.cstart[c + 1];
+    const int size = own_hi - own_lo;
     bool core = size >= ms;  // dbscan_grid.hpp:271-279
     if (!core) {
-      int pop = 0;  // the window's points: fewer than ms, no core
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        int b, e;
-        win(c, k, b, e);
-        pop += s.cstart[e] - s.cstart[b];
-      }
-      if (pop >= ms) {
-        const float xi = s.x[p], yi = s.y[p];
-        int count = size;  // own cell wholesale (dbscan_grid.hpp:293-296)
-        for (int k = 0; k < 5 && count < ms; ++k) {
-          int b, e;
-          win(c, k, b, e);
-          for (int nc = b; nc < e && count < ms; ++nc) {
-            if (nc == c) continue;
-            const int j1 = s.cstart[nc + 1];
-            for (int j = s.cstart[nc]; j < j1; ++j) {
-              const float ex = s.x[j] - xi, ey = s.y[j] - yi;
-              if (ex * ex + ey * ey <= eps_sq) ++count;
-            }
-          }
+      int lo[5], hi[5];
+      const int pop = window(c, lo, hi);
+      if (pop >= ms) {  // else fewer than ms points around: not core
+        if (pop - size > kSmallLaneScan) {
+          pqueue[atomicAdd(&s.qcore, 1)] = static_cast<uint16_t>(p);
+          continue;
         }
+        const float xi = s.x[p], yi = s.y[p];
+        int count = size;
+        for (int k = 0; k < 5; ++k)
+          for (int j = lo[k]; j < hi[k]; ++j)
+            count += (j < own_lo || j >= own_hi) && dist_ok(j, xi, yi) ? 1 : 0;
         core = count >= ms;
       }
     }
@@ -3395,33 +3533,234 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
     if (core) s.has_core[c] = 1;  // equal stores only
   }
   __syncthreads();
-
-  // 6. union-find over reference indices
-  for (int p = t; p < n; p += kSmallThreads) {
-    if (!s.core[p]) continue;
-    const int c = pcell[p];
-    const uint32_t rc = s.cref[c];
-    const float xi = s.x[p], yi = s.y[p];
-    for (int k = 0; k < 5; ++k) {
-      int b, e;
-      win(c, k, b, e);
-      for (int nc = b; nc < e; ++nc) {
-        if (nc <= c || !s.has_core[nc]) continue;  // each cell pair once
-        const uint32_t rn = s.cref[nc];
-        if (small_find(uf, rc, &s.err) == small_find(uf, rn, &s.err)) continue;
-        const int j1 = s.cstart[nc + 1];
-        for (int j = s.cstart[nc]; j < j1; ++j) {
-          if (!s.core[j]) continue;
-          const float ex = s.x[j] - xi, ey = s.y[j] - yi;
-          if (ex * ex + ey * ey <= eps_sq) {
-            small_unite(uf, rc, rn, &s.err);
-            break;
-          }
+  {
+    // groups of kSmallGroup lanes, a queued point each; the count is
+    // order-free, so the nearest columns go first and it reaches ms sooner
+    constexpr int G = kSmallCoreGroup;
+    const int sub = lane & (G - 1), group = t / G;
+    const int nq = __builtin_amdgcn_readfirstlane(s.qcore);
+    for (int q = group; q < nq; q += kSmallThreads / G) {
+      const int p = pqueue[q];
+      const int c = pcell[p];
+      const int own_lo = s.cstart[c], own_hi = s.cstart[c + 1];
+      int lo[5], hi[5];
+      window(c, lo, hi);
+      const float xi = s.x[p], yi = s.y[p];
+      int count = own_hi - own_lo;
+      for (int o = 0; o < 5 && count < ms; ++o) {
+        const int k = near_column(o);
+        for (int j0 = lo[k]; j0 < hi[k] && count < ms; j0 += G) {
+          const int j = j0 + sub;
+          const bool hit = j < hi[k] && (j < own_lo || j >= own_hi) && dist_ok(j, xi, yi);
+          count += static_cast<int>(__popcll(group_bits<G>(__ballot(hit))));
         }
+      }
+      const bool core = count >= ms;
+      if (sub == 0) {
+        s.core[p] = core ? 1 : 0;
+        if (core) s.has_core[c] = 1;
       }
     }
   }
   __syncthreads();
+
+  // 6. union-find over reference indices (dbscan_grid.hpp:351-387): each pair
+  // of core cells in each other's window is tested once, from the lower
+  // cell; CAS hooks the larger root onto the smaller.  A later window cell of
+  // c is one of the next two of its own column or one of the (at most five)
+  // of each column to its right.
+  if (C > kSmallBoxes) {
+    // many cells (a sparse cloud, few points per cell): a lane per cell
+    for (int c = t; c < C; c += kSmallThreads) {
+      if (!s.has_core[c]) continue;
+      const uint32_t rc = s.cref[c];
+      const int c_lo = s.cstart[c], c_hi = s.cstart[c + 1];
+      const uint32_t wl = s.wlen[c];
+      for (int k = 2; k < 5; ++k) {
+        const int b = s.wbase[c][k], e = b + ((wl >> (3 * k)) & 7u);
+        for (int nc = b > c ? b : c + 1; nc < e; ++nc) {
+          if (!s.has_core[nc]) continue;
+          const uint32_t rn = s.cref[nc];
+          if (small_find(uf, rc, &s.err) == small_find(uf, rn, &s.err)) continue;
+          const int n_lo = s.cstart[nc], n_hi = s.cstart[nc + 1];
+          bool found = false;
+          for (int i = c_lo; i < c_hi && !found; ++i) {
+            if (!s.core[i]) continue;
+            const float xi = s.x[i], yi = s.y[i];
+            for (int j = n_lo; j < n_hi; ++j) {
+              if (s.core[j] && dist_ok(j, xi, yi)) {
+                found = true;
+                break;
+              }
+            }
+          }
+          if (found) small_unite(uf, rc, rn, &s.err);
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+    // Few cells, possibly large: the bounding box of each cell's finite core
+    // points (ordered-integer bounds, in the rank and id space, free until
+    // stage 7) prunes as the reference's boxes do (dbscan_grid.hpp:369-377):
+    // a box pair farther apart than eps holds no eps-close pair, and a point
+    // farther than eps from a box has no eps-close point in it (rounding is
+    // monotone, so neither prune drops a pair the exact test would keep).
+    uint4* __restrict__ box = reinterpret_cast<uint4*>(&s.r.u.rank[0]);
+    for (int c = t; c < C; c += kSmallThreads)
+      box[c] = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
+    __syncthreads();
+    {
+      // thread t: sorted points [t * kSmallItems, (t + 1) * kSmallItems),
+      // one set of atomics per run of one cell
+      const int p0 = t * kSmallItems;
+      int cur = -1;
+      uint4 acc = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
+      auto flush = [&]() {
+        if (cur >= 0 && acc.x != 0xFFFFFFFFu) {
+          atomicMin(&box[cur].x, acc.x);
+          atomicMax(&box[cur].y, acc.y);
+          atomicMin(&box[cur].z, acc.z);
+          atomicMax(&box[cur].w, acc.w);
+        }
+      };
+#pragma unroll
+      for (int k = 0; k < kSmallItems; ++k) {
+        const int p = p0 + k;
+        if (p >= n) break;
+        const int c = pcell[p];
+        if (c != cur) {
+          flush();
+          cur = c;
+          acc = make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
+        }
+        const float x = s.x[p], y = s.y[p];
+        if (s.core[p] && isfinite(x) && isfinite(y)) {
+          acc.x = min(acc.x, f2ord(x));
+          acc.y = max(acc.y, f2ord(x));
+          acc.z = min(acc.z, f2ord(y));
+          acc.w = max(acc.w, f2ord(y));
+        }
+      }
+      flush();
+    }
+    __syncthreads();
+    auto box_of = [&](int c) {
+      const uint4 u = box[c];
+      return make_float4(ord2f(u.x), ord2f(u.y), ord2f(u.z), ord2f(u.w));
+    };
+    // pairs a lane leaves undecided go to groups of kSmallPairGroup lanes
+    // (queued in the column table's space, dead now)
+    uint32_t* __restrict__ pq = reinterpret_cast<uint32_t*>(&s.col[0]);
+    constexpr int kPairQueue = (kSmallCols + 8) / 2;
+    // Round 0 takes the pairs at most one column apart (nearly always joined
+    // when both are core), round 1 the rest, most of which round 0 joined.
+    for (int round = 0; round < 2; ++round) {
+      if (wave0) s.qcore = 0;
+      __syncthreads();
+      const int slots = round == 0 ? 7 : 5;
+      for (int item = t; item < slots * C; item += kSmallThreads) {
+        const int c = item / slots, slot = item - slots * c;
+        if (!s.has_core[c]) continue;
+        int nc, e;
+        if (round == 0 && slot < 2) {  // own column
+          e = s.wbase[c][2] + ((s.wlen[c] >> 6) & 7u);
+          nc = c + 1 + slot;
+        } else {
+          const int k = round == 0 ? 3 : 4;
+          const int b = s.wbase[c][k];
+          e = b + ((s.wlen[c] >> (3 * k)) & 7u);
+          nc = b + (round == 0 ? slot - 2 : slot);
+        }
+        if (nc >= e || !s.has_core[nc]) continue;
+        const float4 bn = box_of(nc);
+        if (box_box_d2(box_of(c), bn) > eps_sq) continue;
+        const uint32_t rc = s.cref[c], rn = s.cref[nc];
+        if (small_find(uf, rc, &s.err) == small_find(uf, rn, &s.err)) continue;
+        const int c_lo = s.cstart[c], c_hi = s.cstart[c + 1];
+        const int n_lo = s.cstart[nc], n_hi = s.cstart[nc + 1];
+        // The lane's share: at most kSmallLaneTests steps over (i, j), i
+        // over c's points, j over nc's, branch-free (a point of c that is
+        // not core or is farther than eps from nc's box skips its row); the
+        // wave leaves the loop once none of its lanes has work left.
+        bool found = false;
+        int i = c_lo, j = n_lo;
+        float xi = s.x[i], yi = s.y[i];
+        bool row = s.core[i] && !(point_box_d2(xi, yi, bn) > eps_sq);
+        for (int step = 0; step < kSmallLaneTests; step += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool live = !found && i < c_hi;
+            const int jc = j < n_hi ? j : n_hi - 1;
+            const float ex = s.x[jc] - xi, ey = s.y[jc] - yi;
+            found = found || (live && row && s.core[jc] && ex * ex + ey * ey <= eps_sq);
+            const bool adv = live && !found && (!row || j + 1 >= n_hi);
+            j = adv ? n_lo : (live && !found ? j + 1 : j);
+            i = adv ? i + 1 : i;
+            const int ic = i < c_hi ? i : c_hi - 1;
+            const float nx = s.x[ic], ny = s.y[ic];
+            const bool nrow = s.core[ic] && !(point_box_d2(nx, ny, bn) > eps_sq);
+            xi = adv ? nx : xi;
+            yi = adv ? ny : yi;
+            row = adv ? nrow : row;
+          }
+          if (!__ballot(!found && i < c_hi)) break;
+        }
+        const bool done = found || i >= c_hi;
+        if (found) {
+          small_unite(uf, rc, rn, &s.err);
+        } else if (!done) {
+          const int q = atomicAdd(&s.qcore, 1);
+          if (q < kPairQueue) {
+            pq[q] = (static_cast<uint32_t>(c) << 16) | static_cast<uint32_t>(nc);
+          } else {  // queue full: the lane finishes the pair
+            for (int i = c_lo; i < c_hi && !found; ++i) {
+              if (!s.core[i]) continue;
+              const float xi = s.x[i], yi = s.y[i];
+              if (point_box_d2(xi, yi, bn) > eps_sq) continue;
+              for (int j = n_lo; j < n_hi; ++j) {
+                if (s.core[j] && dist_ok(j, xi, yi)) {
+                  found = true;
+                  break;
+                }
+              }
+            }
+            if (found) small_unite(uf, rc, rn, &s.err);
+          }
+        }
+      }
+      __syncthreads();
+      constexpr int G = kSmallPairGroup;
+      const int sub = lane & (G - 1), group = t / G;
+      const int nq = min(__builtin_amdgcn_readfirstlane(s.qcore), kPairQueue);
+      for (int q = group; q < nq; q += kSmallThreads / G) {
+        const uint32_t v = pq[q];
+        const int c = static_cast<int>(v >> 16), nc = static_cast<int>(v & 0xFFFFu);
+        const uint32_t rc = s.cref[c], rn = s.cref[nc];
+        if (small_find(uf, rc, &s.err) == small_find(uf, rn, &s.err)) continue;
+        const float4 bn = box_of(nc);
+        const int c_lo = s.cstart[c], c_hi = s.cstart[c + 1];
+        const int n_lo = s.cstart[nc], n_hi = s.cstart[nc + 1];
+        bool found = false;
+        for (int i0 = c_lo; i0 < c_hi && !found; i0 += G) {
+          const int i = i0 + sub;
+          bool act = i < c_hi && s.core[i];
+          const float xi = act ? s.x[i] : 0.f, yi = act ? s.y[i] : 0.f;
+          act = act && !(point_box_d2(xi, yi, bn) > eps_sq);
+          if (!group_bits<G>(__ballot(act))) continue;
+          for (int j = n_lo; j < n_hi; ++j) {
+            if (!s.core[j]) continue;  // the same j in every lane of the group
+            if (group_bits<G>(__ballot(act && dist_ok(j, xi, yi)))) {
+              found = true;
+              break;
+            }
+          }
+        }
+        if (found && sub == 0) small_unite(uf, rc, rn, &s.err);
+      }
+      __syncthreads();
+    }
+  }
 
   // 7. cluster ids: every reference index to its root, roots ranked in
   // reference order
@@ -3464,43 +3803,80 @@ __device__ __forceinline__ void small_cloud(SmallLds& s, const CloudInfo& ci, co
   __syncthreads();
   for (int c = t; c < C; c += kSmallThreads)
     s.r.u.ccl[c] = s.has_core[c] ? s.r.u.rank[uf[s.cref[c]]] : kSmallNone;
+  if (wave0) s.qcore = 0;
   __syncthreads();
 
-  // 8. labels, in input order
+  // 8. labels, in input order (dbscan_grid.hpp:403-454).  A core point takes
+  // its cell's id; another the first strict minimum of d2 over the cores
+  // within eps in scan order, i.e. the least (d2, sorted index) (both ascend
+  // with the scan), by its lane for a small window, else by a wave.
+  uint16_t* __restrict__ bqueue = s.r.u.rank;  // ranks are dead now
   const float best_init = ci.best_init;
-  for (int p = t; p < n; p += kSmallThreads) {
-    const int c = pcell[p];
-    int32_t lab;
-    if (s.core[p]) {
-      lab = s.r.u.ccl[c];
-    } else {
-      // dbscan_grid.hpp:419-445: window offsets in order (dx-major, then dy,
-      // which is the column's cell order); a strictly smaller d2 wins
-      lab = kNoise;
-      float best = best_init;
-      const float xi = s.x[p], yi = s.y[p];
-      for (int k = 0; k < 5; ++k) {
-        int b, e;
-        win(c, k, b, e);
-        for (int nc = b; nc < e; ++nc) {
-          if (!s.has_core[nc]) continue;
-          const int32_t cl = s.r.u.ccl[nc];
-          const int j1 = s.cstart[nc + 1];
-          for (int j = s.cstart[nc]; j < j1; ++j) {
-            if (!s.core[j]) continue;
-            const float ex = s.x[j] - xi, ey = s.y[j] - yi;
-            const float d2 = ex * ex + ey * ey;
-            if (d2 <= eps_sq && d2 < best) {
-              best = d2;
-              lab = cl;
-            }
-          }
-        }
-      }
-    }
+  auto store = [&](int p, int32_t lab) {
     const int i = perm[p];
     if (i < n) labels[ci.begin + i] = lab;
     else atomicOr(&s.err, static_cast<int32_t>(kErrRange));
+  };
+  for (int p = t; p < n; p += kSmallThreads) {
+    const int c = pcell[p];
+    if (s.core[p]) {
+      store(p, s.r.u.ccl[c]);
+      continue;
+    }
+    int lo[5], hi[5];
+    if (window(c, lo, hi) > kSmallLaneScan) {
+      bqueue[atomicAdd(&s.qcore, 1)] = static_cast<uint16_t>(p);
+      continue;
+    }
+    const float xi = s.x[p], yi = s.y[p];
+    float best = best_init;
+    int32_t lab = kNoise;
+    for (int k = 0; k < 5; ++k) {
+      for (int j = lo[k]; j < hi[k]; ++j) {
+        if (!s.core[j]) continue;
+        const float ex = s.x[j] - xi, ey = s.y[j] - yi;
+        const float d2 = ex * ex + ey * ey;
+        if (d2 <= eps_sq && d2 < best) {
+          best = d2;
+          lab = s.r.u.ccl[pcell[j]];
+        }
+      }
+    }
+    store(p, lab);
+  }
+  __syncthreads();
+  {
+    const int nq = __builtin_amdgcn_readfirstlane(s.qcore);
+    for (int q = wave; q < nq; q += kSmallWaves) {
+      const int p = __builtin_amdgcn_readfirstlane(bqueue[q]);
+      int lo[5], hi[5];
+      window(pcell[p], lo, hi);
+      const float xi = s.x[p], yi = s.y[p];
+      float best = best_init;
+      int jbest = INT_MAX;
+      for (int k = 0; k < 5; ++k) {
+        for (int j = lo[k] + lane; j < hi[k]; j += 64) {
+          if (!s.core[j]) continue;
+          const float ex = s.x[j] - xi, ey = s.y[j] - yi;
+          const float d2 = ex * ex + ey * ey;
+          if (d2 <= eps_sq && d2 < best) {  // a lane's j ascend: the first wins ties
+            best = d2;
+            jbest = j;
+          }
+        }
+      }
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        const float ob = __shfl_xor(best, d, 64);
+        const int oj = __shfl_xor(jbest, d, 64);
+        if (ob < best || (ob == best && oj < jbest)) {
+          best = ob;
+          jbest = oj;
+        }
+      }
+      jbest = __builtin_amdgcn_readfirstlane(jbest);
+      if (lane == 0) store(p, jbest == INT_MAX ? kNoise : static_cast<int32_t>(s.r.u.ccl[pcell[jbest]]));
+    }
   }
   __syncthreads();
 }
@@ -3746,10 +4122,11 @@ constexpr int kStageSlots[DBSCAN_HIP_NUM_STAGES][2] = {
 // (Widening radix_bits changes nothing unless it saves a pass: the digit
 // width is the key width over the pass count.  At 10M points, 9 bits sort
 // 25-27-bit keys in 3 passes of about 94 us instead of 4 of about 74 us.)
-// Clouds of at most kSmallDefault points take the one-workgroup path: the
-// size below which it beat the pipeline (single calls of constant-density
-// clouds on one MI355X: 58 against 114 us at 100 points, even at 1,000).
-constexpr int kSmallDefault = 1000;
+// Clouds of at most kSmallDefault points take the one-workgroup path: on one
+// MI355X it beats the pipeline at every size it takes (single calls of
+// constant-density clouds: 45 against 113 us at 100 points, 80 against 136
+// us at 4,096; batches of 16 to 1,024 such clouds 1.4-2x).
+constexpr int kSmallDefault = kSmallMax;
 
 dbscan_hip_config default_config(int64_t n) {
   if (n <= 31622) return dbscan_hip_config{8, 8, 2048, 16384, kSmallDefault};

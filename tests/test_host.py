@@ -196,7 +196,7 @@ def test_launch_configuration_checks():
     d = _Cfg()
     assert lib.dbscan2d_hip_default_config(10_000_000, ctypes.byref(d)) == 0
     assert d.sort_items in (8, 12, 16) and 4 <= d.radix_bits <= 9
-    assert 0 < d.small_max <= 4096  # DBSCAN_HIP_SMALL_MAX
+    assert d.small_max == 4096  # DBSCAN_HIP_SMALL_MAX
     for bad in ((10, 8, 8192, 8192), (12, 10, 8192, 8192), (12, 3, 8192, 8192),
                 (12, 8, 10, 8192), (12, 8, 8192, 1 << 20), (12, 8, 8192, 8192, -1),
                 (12, 8, 8192, 8192, 4097)):
@@ -213,16 +213,16 @@ def test_tuning_table_round_trip():
     try:
         autotune.install_rows([{"max_n": 50_000, "config": [8, 7, 2048, 4096]},
                                {"max_n": 2**31 - 1, "config": [16, 8, 8192, 16384]}])
-        # rows without small_max take the library's default (1000)
-        assert _C.get_tuning_table() == [[50_000, 8, 7, 2048, 4096, 1000],
-                                         [2**31 - 1, 16, 8, 8192, 16384, 1000]]
+        # rows without small_max take the library's default (4096)
+        assert _C.get_tuning_table() == [[50_000, 8, 7, 2048, 4096, 4096],
+                                         [2**31 - 1, 16, 8, 8192, 16384, 4096]]
         autotune.install_rows([{"max_n": 50_000, "config": [8, 7, 2048, 4096, 1000]}])
         assert _C.get_tuning_table() == [[50_000, 8, 7, 2048, 4096, 1000]]
         with pytest.raises(RuntimeError, match="tuning rows"):
             _C.set_tuning_table([[1, 2, 3]])
     finally:
         _C.set_tuning_table(saved)
-    assert _C.default_config(1000) == [8, 8, 2048, 16384, 1000]
+    assert _C.default_config(1000) == [8, 8, 2048, 16384, 4096]
 
 
 def test_synthetic_tuning_cloud_is_seeded():
