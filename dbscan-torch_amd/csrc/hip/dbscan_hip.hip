@@ -977,18 +977,20 @@ __global__ __launch_bounds__(kSortThreads, kItems <= 12 ? 4 : 2) void k_sort_pas
 // claimed tiles always make progress whatever the residency.  Every wait is
 // bounded by kSpinLimit.  Pass p reads buffer set a if p is odd, else b.
 // tile_counter[0] is the ticket, tile_counter[1 + q] the tiles done of pass
-// kTailPass + q (cleared by the call's first kernel).
+// first_pass + q (cleared by the call's first kernel).  first_pass is
+// kTailPass, or 1 for inputs of few tiles (kTailFromOneTiles), whose passes
+// cost more in kernel boundaries than in work.
 template <bool kSeg, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
     float* __restrict__ xs_a, float* __restrict__ ys_a, uint32_t* __restrict__ idx_a,
     float* __restrict__ xs_b, float* __restrict__ ys_b, uint32_t* __restrict__ idx_b,
     Params* __restrict__ prm, uint64_t* __restrict__ status, int64_t n64, float inv_r,
     const SegTile* __restrict__ tiles, const CloudInfo* __restrict__ clouds,
-    const uint32_t* __restrict__ cloud_hist, uint32_t num_tiles) {
-  static_assert(kMaxPasses - kTailPass + 1 <= 16, "one done counter per tail pass");
+    const uint32_t* __restrict__ cloud_hist, int first_pass, uint32_t num_tiles) {
+  static_assert(kMaxPasses <= 16, "one done counter per tail pass");
   const int np = prm->npasses;
-  if (np <= kTailPass || num_tiles == 0) return;
-  const uint32_t items = static_cast<uint32_t>(np - kTailPass) * num_tiles;
+  if (np <= first_pass || num_tiles == 0) return;
+  const uint32_t items = static_cast<uint32_t>(np - first_pass) * num_tiles;
   __shared__ uint32_t s_ticket;
   // Every branch around the barriers below is wave-uniform: wave 0 as a
   // whole claims the ticket, waits and counts (lane 0 contributes the only
@@ -1009,7 +1011,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
     __syncthreads();
     if (k >= items) break;
     const int q = static_cast<int>(k / num_tiles);
-    const int pass = kTailPass + q;
+    const int pass = first_pass + q;
     const uint32_t tile = k - static_cast<uint32_t>(q) * num_tiles;
     if (q > 0) {
       if (wave0) {
@@ -4181,18 +4183,29 @@ void launch_sort(int items, int tiles, hipStream_t st, A... a) {
     k_sort_pass<kFirst, kSeg, 12><<<tiles, kSortThreads, 0, st>>>(a..., nt);
 }
 
-// Passes kTailPass.. in one launch (k_sort_tail).
+// Passes first_pass.. in one launch (k_sort_tail).
 template <bool kSeg, class... A>
-void launch_sort_tail(int items, int tiles, hipStream_t st, A... a) {
+void launch_sort_tail(int items, int tiles, int first_pass, hipStream_t st, A... a) {
   const uint32_t nt = static_cast<uint32_t>(tiles);
   const int g = tail_grid(tiles);
   if (items == 8)
-    k_sort_tail<kSeg, 8><<<g, kSortThreads, 0, st>>>(a..., nt);
+    k_sort_tail<kSeg, 8><<<g, kSortThreads, 0, st>>>(a..., first_pass, nt);
   else if (items == 16)
-    k_sort_tail<kSeg, 16><<<g, kSortThreads, 0, st>>>(a..., nt);
+    k_sort_tail<kSeg, 16><<<g, kSortThreads, 0, st>>>(a..., first_pass, nt);
   else
-    k_sort_tail<kSeg, 12><<<g, kSortThreads, 0, st>>>(a..., nt);
+    k_sort_tail<kSeg, 12><<<g, kSortThreads, 0, st>>>(a..., first_pass, nt);
 }
+
+// Inputs of at most this many radix tiles run passes 1.. in the tail launch
+// (a kernel boundary costs more than handing a pass over inside the launch
+// when the passes are this short: 8k-64k points 6-19 us faster per call on
+// one MI355X; at 61 tiles the hand-off was 9 us slower), and inputs of at
+// most kDirectLabelMax points write their labels directly (k_label_direct)
+// instead of through the two bucketed passes.
+constexpr int kTailFromOneTiles = 16;
+constexpr int64_t kDirectLabelMax = int64_t{1} << 18;
+int tail_first_pass(int tiles) { return tiles <= kTailFromOneTiles ? 1 : kTailPass; }
+uint32_t label_buckets(int64_t n) { return n <= kDirectLabelMax ? 0u : bucket_width(n); }
 
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
         void* workspace, size_t workspace_bytes, hipStream_t stream, float* stage_ms,
@@ -4323,17 +4336,18 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   launch_sort<true, false>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
                            ys_a, idx_a, prm, sort_status, 0, n, inv_r, nullptr, nullptr,
                            nullptr);
+  const int tail_from = tail_first_pass(g_sort);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    if (p < kTailPass)
+    if (p < tail_from)
       launch_sort<false, false>(
           cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
           from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
           from_a ? idx_b : idx_a, prm, sort_status, p, n, inv_r, nullptr, nullptr, nullptr);
-    else if (p == kTailPass)
-      launch_sort_tail<false>(cfg.sort_items, g_sort, stream, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b,
-                              prm, sort_status, n, inv_r, nullptr, nullptr, nullptr);
+    else if (p == tail_from)
+      launch_sort_tail<false>(cfg.sort_items, g_sort, tail_from, stream, xs_a, ys_a, idx_a, xs_b,
+                              ys_b, idx_b, prm, sort_status, n, inv_r, nullptr, nullptr, nullptr);
   }
 
   // Stage 3: cell runs.
@@ -4386,7 +4400,7 @@ int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* l
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
                                             ngh, cell_flags, cell_box, is_core, cell_cluster, pp);
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  const uint32_t bw = bucket_width(n);
+  const uint32_t bw = label_buckets(n);
   if (bw > 0) {
     launch_label(bw, n, stream, prm, idx_a, idx_b, point_cell, is_core, cell_cluster, xs_a, xs_b,
                  ys_a, ys_b, sort_status, at<uint32_t>(ws, L.long_runs));
@@ -4574,17 +4588,18 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   launch_sort<true, true>(cfg.sort_items, g_sort, stream, X2, nullptr, nullptr, nullptr, xs_a,
                           ys_a, idx_a, prm, sort_status, 0, n, 0.f, seg_sort, clouds,
                           cloud_hist);
+  const int tail_from = tail_first_pass(g_sort);
   for (int p = 1; p < kMaxPasses; ++p) {
     if ((e = before(kKSort0 + p)) != hipSuccess) return fail(e);
     const bool from_a = (p & 1) != 0;
-    if (p < kTailPass)
+    if (p < tail_from)
       launch_sort<false, true>(
           cfg.sort_items, g_sort, stream, nullptr, from_a ? xs_a : xs_b, from_a ? ys_a : ys_b,
           from_a ? idx_a : idx_b, from_a ? xs_b : xs_a, from_a ? ys_b : ys_a,
           from_a ? idx_b : idx_a, prm, sort_status, p, n, 0.f, seg_sort, clouds, cloud_hist);
-    else if (p == kTailPass)
-      launch_sort_tail<true>(cfg.sort_items, g_sort, stream, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b,
-                             prm, sort_status, n, 0.f, seg_sort, clouds, cloud_hist);
+    else if (p == tail_from)
+      launch_sort_tail<true>(cfg.sort_items, g_sort, tail_from, stream, xs_a, ys_a, idx_a, xs_b,
+                             ys_b, idx_b, prm, sort_status, n, 0.f, seg_sort, clouds, cloud_hist);
   }
   if ((e = before(kKCells)) != hipSuccess) return fail(e);
   k_cells<true><<<g_scan_pts, kScanThreads, 0, stream>>>(
@@ -4624,7 +4639,7 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   k_border<<<g_cellwaves, 256, 0, stream>>>(prm, xs_a, ys_a, idx_a, xs_b, ys_b, idx_b, cell_start,
                                             ngh, cell_flags, cell_box, is_core, cell_cluster, pp);
   if ((e = before(kKLabel)) != hipSuccess) return fail(e);
-  const uint32_t bw = bucket_width(n);
+  const uint32_t bw = label_buckets(n);
   if (bw > 0) {
     launch_label(bw, n, stream, prm, idx_a, idx_b, point_cell, is_core, cell_cluster, xs_a, xs_b,
                  ys_a, ys_b, sort_status, at<uint32_t>(ws, L.long_runs));
