@@ -261,12 +261,19 @@ __device__ __forceinline__ uint64_t cell_key64(uint32_t xr, uint32_t yr) {
   return (static_cast<uint64_t>(xr) << 32) | yr;
 }
 
+// Relaxed agent-scope atomic load and store: coherent across the device's
+// L2s, but ordering nothing else.  Every word published through them is
+// self-contained (a look-back status word carries its tag and its count
+// together; a union-find parent is the whole datum), so readers need no
+// ordering.  Where data other than the word itself must become visible (the
+// sort tail's pass hand-off, the last workgroup of k_small), the code fences
+// explicitly with __threadfence() before the store and after the load.
 template <typename T>
-__device__ __forceinline__ T ld_acquire(const T* p) {
+__device__ __forceinline__ T ld_relaxed(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ void st_release(T* p, T v) {
+__device__ __forceinline__ void st_relaxed(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -319,10 +326,10 @@ __device__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggr
   constexpr uint64_t kAgg = 1ull << 48, kPre = 2ull << 48;
   const int lane = threadIdx.x & 63;
   if (tile == 0) {
-    if (lane == 0) st_release(status, kPre | aggregate);
+    if (lane == 0) st_relaxed(status, kPre | aggregate);
     return 0;
   }
-  if (lane == 0) st_release(status + tile, kAgg | aggregate);
+  if (lane == 0) st_relaxed(status + tile, kAgg | aggregate);
   uint32_t excl = 0;  // counts are below 2^31
   int64_t j = static_cast<int64_t>(tile) - 1;
   uint32_t spins = 0;
@@ -330,7 +337,7 @@ __device__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggr
   for (;;) {
     const int64_t q = j - lane;
     const uint64_t s = lane >= width ? 0ull
-                       : q >= 0      ? ld_acquire(status + q)
+                       : q >= 0      ? ld_relaxed(status + q)
                                      : kPre;  // before tile 0: prefix 0
     const uint64_t tag = s >> 48;
     const uint64_t pre = __ballot(tag == 2), miss = __ballot(tag == 0);
@@ -353,7 +360,7 @@ __device__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggr
       __builtin_amdgcn_s_sleep(1);
     }
   }
-  if (lane == 0) st_release(status + tile, kPre | (static_cast<uint64_t>(excl) + aggregate));
+  if (lane == 0) st_relaxed(status + tile, kPre | (static_cast<uint64_t>(excl) + aggregate));
   return excl;
 }
 
@@ -364,23 +371,23 @@ __device__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggr
 // Read-only root walk, bounded by kSpinLimit steps (a longer walk can only
 // come from corrupted parents: kErrUfLoop, and the walk stops there).
 __device__ __forceinline__ int32_t uf_find_ro(const int32_t* uf, int32_t x, int32_t* error) {
-  int32_t p = ld_acquire(uf + x);
+  int32_t p = ld_relaxed(uf + x);
   for (uint32_t it = 0; p != x; ++it) {
     if (it >= kSpinLimit) {
       atomicOr(error, static_cast<int32_t>(kErrUfLoop));
       break;
     }
     x = p;
-    p = ld_acquire(uf + x);
+    p = ld_relaxed(uf + x);
   }
   return x;
 }
 
 __device__ __forceinline__ int32_t uf_find_halving(int32_t* uf, int32_t x) {
-  int32_t p = ld_acquire(uf + x);
+  int32_t p = ld_relaxed(uf + x);
   while (p != x) {
-    const int32_t gp = ld_acquire(uf + p);
-    if (gp != p) st_release(uf + x, gp);
+    const int32_t gp = ld_relaxed(uf + p);
+    if (gp != p) st_relaxed(uf + x, gp);
     x = p;
     p = gp;
   }
@@ -691,7 +698,7 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 #pragma unroll
     for (int k = 0; k < kSortLookback; ++k)
       w[k] = (j - k >= first)
-                 ? ld_acquire(status + static_cast<uint64_t>(j - k) * stride + digit)
+                 ? ld_relaxed(status + static_cast<uint64_t>(j - k) * stride + digit)
                  : tag_pre;
     // consume the published words in order: `used` of them, `stop` at a prefix
     int used = 0;
@@ -868,7 +875,7 @@ __device__ __forceinline__ void sort_pass_tile(
           s_wcnt[w][d] = dtot[q];
           dtot[q] += c;
         }
-        st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d,
+        st_relaxed(status + static_cast<uint64_t>(tile) * kRadixMax + d,
                    (tile == first ? tag_pre : tag_agg) | dtot[q]);
       }
       dsum += dtot[q];
@@ -905,7 +912,7 @@ __device__ __forceinline__ void sort_pass_tile(
       if (tile > first) {
         excl = digit_lookback(status, static_cast<int64_t>(tile), static_cast<int64_t>(first), d,
                               tag_agg, tag_pre, &prm->error);
-        st_release(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
+        st_relaxed(status + static_cast<uint64_t>(tile) * kRadixMax + d, tag_pre | (excl + dtot[q]));
       }
       s_goff[d] = goff[q] + static_cast<uint32_t>(excl) - dst0[q];
     }
@@ -1017,7 +1024,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
       if (wave0) {
         uint32_t spins = 0;
         while (static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(
-                   ld_acquire(&prm->tile_counter[q])))) < num_tiles) {
+                   ld_relaxed(&prm->tile_counter[q])))) < num_tiles) {
           if (++spins > kSpinLimit) {
             atomicOr(&prm->error, static_cast<int32_t>(kErrSortSpin));
             break;
@@ -2322,7 +2329,7 @@ __global__ __launch_bounds__(256) void k_flatten(Params* __restrict__ prm,
     const int32_t r = cell_ref[c];
     const int32_t root = uf_find_ro(uf, r, &prm->error);
     core_ref[r] = cell_flags[c] & 1u;  // the cell's core bit, by reference index
-    st_release(uf + r, root);
+    st_relaxed(uf + r, root);
   }
 }
 
@@ -2396,7 +2403,7 @@ __global__ __launch_bounds__(256) void k_cluster(Params* __restrict__ prm,
     // value written is a root, so concurrent walks stay on valid ancestors)
     const int32_t r = cell_ref[c];
     const int32_t root = uf_find_ro(uf, r, &prm->error);
-    if (root != r) st_release(uf + r, root);
+    if (root != r) st_relaxed(uf + r, root);
     if (cell_flags[c] & 1u) {
       id = static_cast<int32_t>(rank[root]);
       if constexpr (kSeg) id -= static_cast<int32_t>(rank[clouds[cell_cloud[c]].cell_begin]);
@@ -3942,7 +3949,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small(
     __syncthreads();
     if (__builtin_amdgcn_readfirstlane(s.ticket) != static_cast<int32_t>(gridDim.x) - 1) return;
     __threadfence();
-    bits = __builtin_amdgcn_readfirstlane(ld_acquire(&prm->error));
+    bits = __builtin_amdgcn_readfirstlane(ld_relaxed(&prm->error));
   }
   if (bits) {
     for (int64_t i = threadIdx.x; i < total; i += kSmallThreads) labels[i] = kPoison;
