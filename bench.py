@@ -23,6 +23,9 @@ min_samples 10, seed 2).  Prints one JSON line:
   cpu_baseline   the reference's own CPU path (compiled from its sources into
                  oracle/_ref) on the same workload, on the host cores, or the C
                  oracle where that build is absent
+  small          clouds of at most 4,096 points: latency of the one-workgroup
+                 path (one launch; a persistent grid for an all-small batch)
+                 against the pipeline forced on the same inputs
   batched        2,048 clouds (~135M points, per-cloud eps and min_samples):
                  clouds/s of one batched call against a Python loop of plain
                  dbscan2d calls over the same resident clouds, labels compared
@@ -290,6 +293,69 @@ SERVER_CONFIG = dict(slots=3, slot_points=8 << 20, slot_clouds=4096, big_slots=3
                      big_points=24 << 20, big_threshold=1 << 20)
 
 
+def small_leg(dbscan_torch) -> dict:
+    """Clouds of at most 4,096 points: the one-workgroup path (one launch per
+    call, or one persistent grid per all-small batch) against the multi-kernel
+    pipeline forced on the same inputs (small_max = 0).  Median latency of a
+    synchronised call; labels must be identical."""
+    import numpy as np
+    import torch
+
+    from tests import workloads as W
+
+    C = dbscan_torch._C
+
+    def cfg(n, small):
+        c = list(C.default_config(n))
+        c[4] = small
+        return c
+
+    def latency_us(fn, reps=100):
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return round(float(np.median(ts)) * 1e6, 1)
+
+    out = {"workload": "clouds at blobs10k density (cloud_at_density), eps 0.5, min_samples 10; "
+                       "batch: 1,024 clouds, N log-uniform in [200, 4096], eps U[0.2, 1.0], "
+                       "min_samples {5, 10, 20}",
+           "single": {}}
+    for n in (1000, 4096):
+        Xd = torch.from_numpy(W.cloud_at_density(n, 77)).cuda()
+        one = lambda: C.dbscan2d_configured(Xd, 0.5, 10, cfg(n, 4096))  # noqa: E731
+        pipe = lambda: C.dbscan2d_configured(Xd, 0.5, 10, cfg(n, 0))  # noqa: E731
+        out["single"][str(n)] = {"one_workgroup_us": latency_us(one),
+                                 "pipeline_us": latency_us(pipe),
+                                 "labels_identical": bool(torch.equal(one(), pipe()))}
+    rng = np.random.default_rng(1)
+    ns = np.floor(np.exp(rng.uniform(np.log(200), np.log(4096), 1024))).astype(int)
+    X = torch.from_numpy(np.concatenate([W.cloud_at_density(int(k), 500 + i)
+                                         for i, k in enumerate(ns)])).cuda()
+    offs = [0] + np.cumsum(ns).tolist()
+    eps = rng.uniform(0.2, 1.0, len(ns)).tolist()
+    ms = rng.choice([5, 10, 20], len(ns)).tolist()
+    saved = C.get_tuning_table()
+    try:
+        C.set_tuning_table([[2**31 - 1] + cfg(int(offs[-1]), 4096)])
+        grid = C.dbscan2d_batched(X, offs, eps, ms)
+        t_grid = latency_us(lambda: C.dbscan2d_batched(X, offs, eps, ms), 30)
+        C.set_tuning_table([[2**31 - 1] + cfg(int(offs[-1]), 0)])
+        pipe = C.dbscan2d_batched(X, offs, eps, ms)
+        t_pipe = latency_us(lambda: C.dbscan2d_batched(X, offs, eps, ms), 30)
+    finally:
+        C.set_tuning_table(saved)
+    out["batch_1024"] = {"points": int(offs[-1]), "persistent_grid_ms": round(t_grid / 1e3, 3),
+                         "pipeline_ms": round(t_pipe / 1e3, 3),
+                         "labels_identical": bool(torch.equal(grid, pipe))}
+    return out
+
+
 def serving_leg(dbscan_torch, rates, count: int, pool: int = 384) -> dict:
     """Open-loop Poisson request streams served by the continuous-batching
     server (pooled arena: 3 slots of 8M points for requests below 1M points,
@@ -386,6 +452,10 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-vs-looped leg (prints its JSON)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small-cloud leg (one workgroup against the pipeline)")
+    ap.add_argument("--small-only", action="store_true",
+                    help="run only the small-cloud leg (prints its JSON)")
     ap.add_argument("--serving-only", action="store_true",
                     help="run only the serving leg (prints its JSON)")
     ap.add_argument("--serve-rates", default="100,250,500,1000,2000,5000",
@@ -422,6 +492,9 @@ def main() -> int:
     if args.serving_only:
         print(json.dumps({"serving": serving_leg(dbscan_torch, rates, args.serve_requests)}),
               flush=True)
+        return 0
+    if args.small_only:
+        print(json.dumps({"small": small_leg(dbscan_torch)}), flush=True)
         return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(dbscan_torch, max(1, min(args.steps, 5)),
@@ -505,6 +578,9 @@ def main() -> int:
             print(f"[bench] {name}: {out['configs'][name]}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, eps, ms)
+    if not args.no_small:
+        out["small"] = small_leg(dbscan_torch)
+        print(f"[bench] small: {out['small']}", file=sys.stderr, flush=True)
     if not args.no_batched:
         torch.cuda.empty_cache()
         out["batched"] = batched_leg(dbscan_torch, 3, 1)
@@ -514,6 +590,10 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     bad = [k for k, v in [(args.workload, labels_ok)] +
            [(k, v["labels_ok"]) for k, v in out.get("configs", {}).items()] if v is False]
+    small = out.get("small")
+    if small and not (all(v["labels_identical"] for v in small["single"].values()) and
+                      small["batch_1024"]["labels_identical"]):
+        bad.append("small")
     if bad:
         print(f"bench.py: labels differ from the reference digest for {bad}", file=sys.stderr)
         return 3

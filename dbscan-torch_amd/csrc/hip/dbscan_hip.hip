@@ -4131,6 +4131,10 @@ constexpr int kStageSlots[DBSCAN_HIP_NUM_STAGES][2] = {
 // (Widening radix_bits changes nothing unless it saves a pass: the digit
 // width is the key width over the pass count.  At 10M points, 9 bits sort
 // 25-27-bit keys in 3 passes of about 94 us instead of 4 of about 74 us.)
+// An all-small batch runs as one persistent grid while it has at most this
+// many clouds per CU.
+constexpr int kSmallGridClouds = 6;
+
 // Clouds of at most kSmallDefault points take the one-workgroup path: on one
 // MI355X it beats the pipeline at every size it takes (single calls of
 // constant-density clouds: 45 against 113 us at 100 points, 80 against 136
@@ -4501,15 +4505,25 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   const float2* X2 = reinterpret_cast<const float2*>(X);
   Params* prm = at<Params>(ws, L.params);
 
-  // Every cloud small: a persistent grid of one-workgroup clusterings that
-  // takes the clouds from a device ticket (the per-kernel timing variant
-  // keeps the pipeline).
-  bool all_small = kev == nullptr && cfg.small_max > 0;
+  // Every cloud small, and at most kSmallGridClouds per CU: a persistent grid
+  // of one-workgroup clusterings that takes the clouds from a device ticket
+  // (the per-kernel timing variant keeps the pipeline).  Past that many
+  // clouds per CU the pipeline catches up (on one MI355X, clouds of 200-4,096
+  // points: 64 clouds 0.15 against 0.29 ms, 1,024 clouds 0.35 against 0.46,
+  // 2,048 clouds 0.62 against 0.63).
+  bool all_small = kev == nullptr && cfg.small_max > 0 && B <= kSmallGridClouds * cu_count();
   for (int32_t i = 0; i < B && all_small; ++i)
     all_small = info[i].end - info[i].begin <= cfg.small_max;
   if (all_small) {
+    // Largest clouds first (each cloud writes its labels at its own offset,
+    // so the order is free): the workgroups that draw the last tickets then
+    // get the shortest clouds, and the grid finishes together.
+    std::vector<CloudInfo> order(info);
+    std::stable_sort(order.begin(), order.end(), [](const CloudInfo& a, const CloudInfo& b) {
+      return a.end - a.begin > b.end - b.begin;
+    });
     CloudInfo* clouds = at<CloudInfo>(ws, L.clouds);
-    if ((e = hipMemcpyAsync(clouds, info.data(), sizeof(CloudInfo) * B, hipMemcpyHostToDevice,
+    if ((e = hipMemcpyAsync(clouds, order.data(), sizeof(CloudInfo) * B, hipMemcpyHostToDevice,
                             stream)) != hipSuccess)
       return fail(e);
     const int grid = static_cast<int>(std::min<int64_t>(B, cu_count()));

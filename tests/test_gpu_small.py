@@ -169,6 +169,21 @@ def test_all_small_batch_matches_oracle(dt, count):
         assert np.array_equal(o.cpu().numpy(), oracle.dbscan2d(c, eps[i], ms[i])), i
 
 
+def test_many_small_clouds_use_the_pipeline_and_match(dt):
+    """More small clouds than the persistent grid takes (six per CU): the
+    batch runs the pipeline, with the same labels."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    count = 6 * cus + 17
+    rng = np.random.default_rng(77)
+    ns = rng.integers(1, 40, count)
+    clouds = [cloud(KINDS[i % len(KINDS)], int(k), 5000 + i) for i, k in enumerate(ns)]
+    eps = [EPS[KINDS[i % len(KINDS)]] for i in range(count)]
+    ms = rng.choice([1, 2, 3], count).tolist()
+    out = dt.dbscan2d_batched([torch.from_numpy(c).to(DEV) for c in clouds], eps, ms)
+    for i, (c, o) in enumerate(zip(clouds, out)):
+        assert np.array_equal(o.cpu().numpy(), oracle.dbscan2d(c, eps[i], ms[i])), i
+
+
 def test_mixed_batch_uses_the_pipeline_and_matches(dt):
     clouds, eps, ms = _batch(40, 99)
     clouds.append(cloud("blobs", 6000, 5))
