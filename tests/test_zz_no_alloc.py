@@ -1,7 +1,8 @@
 """The allocation-free entry point (dbscan2d_hip_noalloc) on the GPU, through
 the C-ABI: its labels equal those of dbscan2d_hip (the workspace entry point)
-on two calls in a row, the second on a smaller input, and the oracle's; sizes
-above its limit are refused before any device work."""
+on three calls in a row, each on a smaller input than the one before (the
+last small enough for the one-workgroup path), and the oracle's; sizes above
+its limit are refused before any device work."""
 
 import ctypes
 import os
@@ -56,7 +57,8 @@ def test_noalloc_equals_workspace_entry_point(lib):
     stream = torch.cuda.current_stream().cuda_stream
     big = W.G(1 << _max_points(), 120, (0, 400), 1.0, 0.1, (0, 400), 31)  # the largest allowed
     small = W.G(30_000, 10, (-15, 15), 0.6, 0.1, (-18, 18), 32)
-    for X, eps, ms in ((big, 0.5, 10), (small, 0.4, 6)):
+    tiny = W.G(3_000, 4, (-5, 5), 0.4, 0.1, (-6, 6), 33)  # one-workgroup path
+    for X, eps, ms in ((big, 0.5, 10), (small, 0.4, 6), (tiny, 0.3, 5)):
         n = len(X)
         print(f"[gpu noalloc] n={n} eps={eps} ms={ms}", flush=True)
         Xd = torch.from_numpy(X).to(DEV)
