@@ -1692,13 +1692,13 @@ __device__ __forceinline__ uint32_t lane_min_core_d2(const float* __restrict__ x
   }
   return m;
 }
-// A cell pair whose point-count product is at most this is tested by one lane.
-constexpr int32_t kLanePairs = 1024;
-
-// A first try for a bigger pair by one lane: the first kSampleSide points of
+// A first try for a cell pair by one lane: the first kSampleSide points of
 // each cell against each other (cores only).  A hit is a core pair within eps,
 // which decides the pair; otherwise the whole wave tests it.  Adjacent cells
-// inside a dense region nearly always connect at the first try.
+// inside a dense region nearly always connect at the first try.  Small pairs
+// take the same route: one lane scanning a whole pair (up to 1,024 point
+// pairs) holds its wave for its slowest lane, and on one MI355X that rule was
+// slower on every workload measured (the 2,048-cloud batch by 5% end to end).
 constexpr int kSampleSide = 4;
 
 __device__ __forceinline__ bool core_pair_sample(const float* __restrict__ xs,
@@ -2054,49 +2054,13 @@ __device__ bool core_pair_within(const float* __restrict__ xs, const float* __re
   return false;
 }
 
-// The same test for one lane (small cell pairs).
-__device__ __forceinline__ bool core_pair_within_lane(const float* __restrict__ xs,
-                                                      const float* __restrict__ ys,
-                                                      const uint8_t* __restrict__ is_core,
-                                                      int32_t sa, int32_t ea, int32_t sb,
-                                                      int32_t eb, float4 box_b, float eps_sq) {
-  for (int32_t a = sa; a < ea; ++a) {
-    if (!is_core[a]) continue;
-    const float xa = xs[a], ya = ys[a];
-    if (point_box_d2(xa, ya, box_b) > eps_sq) continue;
-    int32_t b = sb;
-    for (; b + 4 <= eb; b += 4) {
-      float px[4], py[4];
-      uint8_t pc[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        px[u] = xs[b + u];
-        py[u] = ys[b + u];
-        pc[u] = is_core[b + u];
-      }
-      bool hit = false;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float ex = px[u] - xa, ey = py[u] - ya;
-        hit |= pc[u] && ex * ex + ey * ey <= eps_sq;
-      }
-      if (hit) return true;
-    }
-    for (; b < eb; ++b) {
-      if (!is_core[b]) continue;
-      const float ex = xs[b] - xa, ey = ys[b] - ya;
-      if (ex * ex + ey * ey <= eps_sq) return true;
-    }
-  }
-  return false;
-}
-
 // Cell-level union over eps-close core pairs (dbscan_grid.hpp:351-387).  One
 // lane per (cell c, forward neighbour slot): each unordered window pair is
 // visited once, from its smaller internal index (the rest of c's column above
 // c, and the two columns to the right; at most 12 slots).  Pairs with a core
-// on both sides and boxes within eps are tested by the lane when small, and by
-// the whole wave otherwise.
+// on both sides and boxes within eps are sampled by the lane
+// (core_pair_sample) and, when that does not decide them, tested by the whole
+// wave.
 // Two launches split the pairs: kPhase 0 takes the 3x3 neighbours (dx <= 1,
 // |dy| <= 1; only slots 0-6 can hold one), which join almost every dense
 // region, and only records each hit as bit `slot` of the cell's mask (the
@@ -2172,13 +2136,8 @@ __global__ __launch_bounds__(256) void k_union(
       if (box_box_d2(box_c, box_n) <= eps_sq) {
         const int32_t sa = cell_start[c], ea = cell_start[c + 1];
         const int32_t sb = cell_start[nc], eb = cell_start[nc + 1];
-        bool hit = false;
-        if (static_cast<int64_t>(ea - sa) * (eb - sb) <= kLanePairs)
-          hit = core_pair_within_lane(xs, ys, is_core, sa, ea, sb, eb, box_n, eps_sq);
-        else if (core_pair_sample(xs, ys, is_core, sa, ea, sb, eb, eps_sq))
-          hit = true;
-        else
-          deferred = true;
+        const bool hit = core_pair_sample(xs, ys, is_core, sa, ea, sb, eb, eps_sq);
+        deferred = !hit;
         if (hit) {
           if constexpr (kPhase == 0)
             atomicOr(hits + c, 1u << slot);
