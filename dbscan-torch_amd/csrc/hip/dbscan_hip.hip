@@ -2393,12 +2393,19 @@ constexpr int kLabelTile = kLabelThreads * 8;  // the smaller tile of the two fo
 // two words, input position and label.  The form is read from num_roots on
 // the device, in the kernels themselves.
 constexpr int32_t kPackedRoots = (1 << 17) - 1;
+// One-word pairs go in 12,288-point tiles (48 KB of the 64 KB buffer): at 1M
+// to 10M points that was 0.6-1.6% faster end to end than 16,384-point tiles
+// on one MI355X (a 10M-point call has 814 tiles instead of 611 for the 512
+// workgroup slots, so its last round is less idle).
 template <bool kPacked>
 struct LabelForm {
-  static constexpr int kItems = kPacked ? 16 : 8;
+  static constexpr int kItems = kPacked ? 12 : 8;
   static constexpr int kTile = kLabelThreads * kItems;
-  using Slot = std::conditional_t<kPacked, uint32_t, uint64_t>;  // 64 KB of LDS either way
+  using Slot = std::conditional_t<kPacked, uint32_t, uint64_t>;  // at most 64 KB of LDS
 };
+static_assert(LabelForm<true>::kTile * sizeof(uint32_t) <= kLabelTile * sizeof(uint64_t) &&
+                  LabelForm<true>::kTile >= kLabelTile,
+              "one-word tiles fit the pair buffer and need no more run-table columns");
 static_assert(kLabelThreads >= kMaxBuckets, "a thread per bucket");
 // A run of more pairs than this marks its bucket for the chunked unpermute.
 constexpr uint32_t kLongRun = 256;
