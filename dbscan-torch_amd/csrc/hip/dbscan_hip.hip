@@ -1950,6 +1950,19 @@ __device__ int32_t wave_core_count(const float* __restrict__ xs, const float* __
 }
 
 
+// Cells a wave takes per round in the per-cell passes (one lane each): 64
+// when there are enough cells to keep the GPU's wave slots (kWaveSlots,
+// about 256 CUs x 32) busy with full rounds, else a smaller power of two,
+// down to 8.  The pass-2 points of a round are shared out over all 64 lanes
+// whatever the count, so a small input's points spread over many waves
+// instead of queueing in a few; past that size, fewer cells per wave only
+// leave lanes idle.
+constexpr int64_t kWaveSlots = 8192;
+__device__ __forceinline__ int64_t cells_per_wave(int64_t C) {
+  const int64_t per = C / kWaveSlots;
+  return per > 32 ? 64 : per > 16 ? 32 : per > 8 ? 16 : 8;
+}
+
 // Core marking pass 2 (dbscan_grid.hpp:281-307) for the cells with fewer than
 // min_samples points (pass 1 was done by k_bbox).  One lane per cell counts
 // for each of the cell's points in turn; the counts stop at min_samples,
@@ -1970,9 +1983,10 @@ __global__ __launch_bounds__(256) void k_core_sparse(
   const int lane = threadIdx.x & 63;
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
-  for (int64_t base = wave * 64; base < C; base += nwaves * 64) {
+  const int64_t K = cells_per_wave(C);
+  for (int64_t base = wave * K; base < C; base += nwaves * K) {
     const int32_t c = static_cast<int32_t>(base + lane);
-    const bool valid = base + lane < C;
+    const bool valid = lane < K && base + lane < C;
     int32_t s = 0, e = 0;
     CellParams cp{0.f, 0.f, 0};
     if (valid) {
@@ -2954,9 +2968,10 @@ __global__ __launch_bounds__(256) void k_border(
   const int lane = threadIdx.x & 63;
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
-  for (int64_t base = wave * 64; base < C; base += nwaves * 64) {
+  const int64_t K = cells_per_wave(C);
+  for (int64_t base = wave * K; base < C; base += nwaves * K) {
     const int32_t c = static_cast<int32_t>(base + lane);
-    const bool valid = base + lane < C;
+    const bool valid = lane < K && base + lane < C;
     int32_t s = 0, e = 0;
     bool has = false;
     CellParams cp{0.f, 0.f, 0};
