@@ -985,7 +985,7 @@ __global__ __launch_bounds__(kSortThreads, kItems <= 12 ? 4 : 2) void k_sort_pas
 // bounded by kSpinLimit.  Pass p reads buffer set a if p is odd, else b.
 // tile_counter[0] is the ticket, tile_counter[1 + q] the tiles done of pass
 // first_pass + q (cleared by the call's first kernel).  first_pass is
-// kTailPass, or 1 for inputs of few tiles (kTailFromOneTiles), whose passes
+// kTailPass, or 2 for inputs of few tiles (kTailFromOneTiles), whose passes
 // cost more in kernel boundaries than in work.
 template <bool kSeg, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_sort_tail(
@@ -4259,15 +4259,19 @@ void launch_sort_tail(int items, int tiles, int first_pass, hipStream_t st, A...
     k_sort_tail<kSeg, 12><<<g, kSortThreads, 0, st>>>(a..., first_pass, nt);
 }
 
-// Inputs of at most this many radix tiles run passes 1.. in the tail launch
-// (a kernel boundary costs more than handing a pass over inside the launch
-// when the passes are this short: 8k-64k points 6-19 us faster per call on
-// one MI355X; at 61 tiles the hand-off was 9 us slower), and inputs of at
-// most kDirectLabelMax points write their labels directly (k_label_direct)
-// instead of through the two bucketed passes.
+// Inputs of at most this many radix tiles run pass 1 as a plain launch and
+// passes 2.. in the tail launch (a kernel boundary costs more than handing a
+// pass over inside the launch when the passes are this short; at 61 tiles
+// the hand-off was 9 us slower).  Pass 1 stays a plain launch because a
+// small cloud's key usually needs only two passes, and a plain pass is
+// cheaper than the tail's ticketed one while a tail with no pass to run
+// costs nothing measurable (10k-60k points 1-5% faster per call on one
+// MI355X than the tail from pass 1).  Inputs of at most kDirectLabelMax
+// points write their labels directly (k_label_direct) instead of through the
+// two bucketed passes.
 constexpr int kTailFromOneTiles = 16;
 constexpr int64_t kDirectLabelMax = int64_t{1} << 18;
-int tail_first_pass(int tiles) { return tiles <= kTailFromOneTiles ? 1 : kTailPass; }
+int tail_first_pass(int tiles) { return tiles <= kTailFromOneTiles ? 2 : kTailPass; }
 uint32_t label_buckets(int64_t n) { return n <= kDirectLabelMax ? 0u : bucket_width(n); }
 
 int run(const float* X, int64_t n, double eps, int64_t min_samples64, int32_t* labels,
