@@ -4195,8 +4195,9 @@ constexpr int kSmallDefault = kSmallMax;
 
 dbscan_hip_config default_config(int64_t n) {
   if (n <= 31622) return dbscan_hip_config{8, 8, 2048, 16384, kSmallDefault};
-  if (n <= 316227) return dbscan_hip_config{8, 8, 16384, 4096, kSmallDefault};
-  if (n <= 3162277) return dbscan_hip_config{8, 7, 8192, 8192, kSmallDefault};
+  if (n <= 316227) return dbscan_hip_config{8, 9, 16384, 4096, kSmallDefault};
+  if (n <= 2000000) return dbscan_hip_config{8, 9, 8192, 8192, kSmallDefault};
+  if (n <= 3162277) return dbscan_hip_config{12, 9, 8192, 8192, kSmallDefault};
   if (n <= (int64_t{1} << 24)) return dbscan_hip_config{12, 9, 4096, 8192, kSmallDefault};
   return dbscan_hip_config{12, 9, 8192, 8192, kSmallDefault};  // mostly batches of many clouds
 }
@@ -4259,10 +4260,11 @@ void launch_sort_tail(int items, int tiles, int first_pass, hipStream_t st, A...
     k_sort_tail<kSeg, 12><<<g, kSortThreads, 0, st>>>(a..., first_pass, nt);
 }
 
-// Inputs of at most this many radix tiles run pass 1 as a plain launch and
-// passes 2.. in the tail launch (a kernel boundary costs more than handing a
-// pass over inside the launch when the passes are this short; at 61 tiles
-// the hand-off was 9 us slower).  Pass 1 stays a plain launch because a
+// Inputs of at most this many radix tiles (160k points at 4,096-point
+// tiles) run pass 1 as a plain launch and passes 2.. in the tail launch (a
+// kernel boundary costs more than handing a pass over inside the launch
+// when the passes are this short: 100k points, 25 tiles, 4.5% faster per
+// call than with a limit of 16 tiles; at 61 tiles the hand-off was slower).  Pass 1 stays a plain launch because a
 // small cloud's key usually needs only two passes, and a plain pass is
 // cheaper than the tail's ticketed one while a tail with no pass to run
 // costs nothing measurable (10k-60k points 1-5% faster per call on one
@@ -4272,7 +4274,7 @@ void launch_sort_tail(int items, int tiles, int first_pass, hipStream_t st, A...
 // the scattered 4-byte writes cost less than the second pass (300k points 9%
 // faster per call, 1.07M points 3%; at 10M points the direct write is 21%
 // slower).
-constexpr int kTailFromOneTiles = 16;
+constexpr int kTailFromOneTiles = 40;
 constexpr int64_t kDirectLabelMax = int64_t{3} << 19;  // 1,572,864
 int tail_first_pass(int tiles) { return tiles <= kTailFromOneTiles ? 2 : kTailPass; }
 uint32_t label_buckets(int64_t n) { return n <= kDirectLabelMax ? 0u : bucket_width(n); }
