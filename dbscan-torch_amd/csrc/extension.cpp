@@ -578,6 +578,87 @@ void dbscan2d_batched_out(torch::Tensor X, std::vector<int64_t> offsets, std::ve
       stream, &bad, config_for(n), err));
 }
 
+// dbscan2d_batched in two steps, for HIP graph capture (no reference
+// counterpart).  dbscan2d_batched_stage validates the batch (X gives the
+// shape and device), copies its tables into `workspace` on the current
+// stream and returns the launch plan as ten integers; dbscan2d_batched_enqueue
+// enqueues the kernels only, on the current stream (capturable), into
+// `labels`, writing the device error word to `error_word` if given.
+namespace {
+constexpr size_t kPlanInts = 10;
+std::vector<int64_t> plan_to_ints(const dbscan_hip_batch_plan& p) {
+  return {p.total_points, p.num_clouds,   p.sort_tiles,     p.scan_tiles,     p.all_small,
+          p.cfg.sort_items, p.cfg.radix_bits, p.cfg.cell_blocks, p.cfg.pair_blocks, p.cfg.small_max};
+}
+dbscan_hip_batch_plan plan_from_ints(const std::vector<int64_t>& v) {
+  TORCH_CHECK(v.size() == kPlanInts, "dbscan2d_batched_enqueue: a plan has ", kPlanInts,
+              " values (from dbscan2d_batched_stage)");
+  dbscan_hip_batch_plan p{};
+  p.total_points = v[0];
+  p.num_clouds = v[1];
+  p.sort_tiles = v[2];
+  p.scan_tiles = v[3];
+  p.all_small = static_cast<int32_t>(v[4]);
+  p.cfg = dbscan_hip_config{static_cast<int32_t>(v[5]), static_cast<int32_t>(v[6]),
+                            static_cast<int32_t>(v[7]), static_cast<int32_t>(v[8]),
+                            static_cast<int32_t>(v[9])};
+  return p;
+}
+void check_batch_buffers(const torch::Tensor& X, const torch::Tensor& workspace) {
+  TORCH_CHECK(X.is_cuda() && workspace.is_cuda() && workspace.device() == X.device(),
+              "dbscan2d_batched: X and workspace must be on one ROCm device");
+  TORCH_CHECK(X.is_contiguous() && workspace.is_contiguous() &&
+                  workspace.scalar_type() == torch::kUInt8,
+              "dbscan2d_batched: X must be contiguous and workspace a contiguous uint8 tensor");
+}
+}  // namespace
+
+std::vector<int64_t> dbscan2d_batched_stage(torch::Tensor X, std::vector<int64_t> offsets,
+                                            std::vector<double> eps,
+                                            std::vector<int64_t> min_samples,
+                                            torch::Tensor workspace) {
+  const Batch b = check_batch(X, offsets, eps, min_samples);
+  check_batch_buffers(X, workspace);
+  const int64_t n = X.size(0), B = static_cast<int64_t>(eps.size());
+  c10::DeviceGuard guard(X.device());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  dbscan_hip_batch_plan plan{};
+  int bad = -1;
+  check_status(dbscan2d_hip_batched_stage(n, b.offsets.data(), b.eps.data(), b.ms.data(), B,
+                                          workspace.data_ptr(),
+                                          static_cast<size_t>(workspace.numel()), stream, &bad,
+                                          config_for(n), &plan));
+  return plan_to_ints(plan);
+}
+
+void dbscan2d_batched_enqueue(torch::Tensor X, torch::Tensor labels, torch::Tensor workspace,
+                              std::vector<int64_t> plan, std::optional<torch::Tensor> error_word) {
+  const dbscan_hip_batch_plan p = plan_from_ints(plan);
+  check_batch_buffers(X, workspace);
+  TORCH_CHECK(X.dim() == 2 && X.size(1) == 2 && X.scalar_type() == torch::kFloat32 &&
+                  X.size(0) == p.total_points,
+              "dbscan2d_batched_enqueue: X must be the staged batch's float32 (N, 2) points");
+  TORCH_CHECK(labels.is_cuda() && labels.device() == X.device() && labels.is_contiguous() &&
+                  labels.scalar_type() == torch::kInt32 && labels.numel() >= p.total_points,
+              "dbscan2d_batched_enqueue: labels must be a contiguous int32 tensor of at least N "
+              "elements on X's device");
+  int32_t* err = nullptr;
+  if (error_word) {
+    const torch::Tensor& e = *error_word;
+    TORCH_CHECK(e.scalar_type() == torch::kInt32 && e.numel() >= 1 && e.is_contiguous() &&
+                    (e.is_cuda() || e.is_pinned()),
+                "dbscan2d_batched_enqueue: error_word must be an int32 tensor in device or "
+                "pinned host memory");
+    err = e.data_ptr<int32_t>();
+  }
+  c10::DeviceGuard guard(X.device());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(X.device().index()).stream();
+  check_status(dbscan2d_hip_batched_enqueue(X.data_ptr<float>(), labels.data_ptr<int32_t>(),
+                                            workspace.data_ptr(),
+                                            static_cast<size_t>(workspace.numel()), stream, &p,
+                                            err));
+}
+
 int64_t dbscan2d_batched_workspace_bytes(int64_t total_points, int64_t num_clouds) {
   size_t bytes = 0;
   check_status(dbscan2d_hip_batched_workspace_bytes(total_points, num_clouds, &bytes));
@@ -786,6 +867,14 @@ PYBIND11_MODULE(_C, m) {
         pybind11::arg("rows"),
         "Serving helper: clouds copied into consecutive rows of a device tensor, one "
         "asynchronous copy each on the current stream.");
+  m.def("dbscan2d_batched_stage", &dbscan2d_batched_stage, pybind11::arg("X"),
+        pybind11::arg("offsets"), pybind11::arg("eps"), pybind11::arg("min_samples"),
+        pybind11::arg("workspace"),
+        "Stages a batch's tables in `workspace` (current stream); returns the launch plan.");
+  m.def("dbscan2d_batched_enqueue", &dbscan2d_batched_enqueue, pybind11::arg("X"),
+        pybind11::arg("labels"), pybind11::arg("workspace"), pybind11::arg("plan"),
+        pybind11::arg("error_word") = pybind11::none(),
+        "Enqueues a staged batch's kernels only, on the current stream (capturable).");
   m.def("dbscan2d_batched_workspace_bytes", &dbscan2d_batched_workspace_bytes,
         pybind11::arg("total_points"), pybind11::arg("num_clouds"),
         "Workspace bytes of a batch of num_clouds clouds holding total_points points.");

@@ -4523,11 +4523,19 @@ bool cloud_params(double eps, int64_t min_samples64, CloudInfo& ci) {
   return true;
 }
 
-int run_batched(const float* X, int64_t total, const int64_t* offsets, const double* eps,
-                const int64_t* min_samples, int64_t num_clouds, int32_t* labels,
-                void* workspace, size_t workspace_bytes, hipStream_t stream, hipEvent_t* kev,
-                int* bad_cloud, const dbscan_hip_config* cfgp, int32_t* error_out = nullptr) {
+// A batch's host-side tables: per-cloud records (validated exactly as a single
+// call validates its arguments) and the tile counts the launches need.
+struct BatchTables {
+  std::vector<CloudInfo> info;
+  dbscan_hip_batch_plan plan;
+};
+
+int build_batch(int64_t total, const int64_t* offsets, const double* eps,
+                const int64_t* min_samples, int64_t num_clouds, int* bad_cloud,
+                const dbscan_hip_config* cfgp, BatchTables& bt) {
   if (bad_cloud) *bad_cloud = -1;
+  dbscan_hip_batch_plan& plan = bt.plan;
+  std::memset(&plan, 0, sizeof(plan));
   const dbscan_hip_config cfg = cfgp ? *cfgp : default_config(total);
   if (!config_ok(cfg)) return DBSCAN_HIP_ERR_CONFIG;
   const int32_t sort_tile = kSortThreads * cfg.sort_items;
@@ -4536,11 +4544,11 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   if (num_clouds == 0) return total == 0 ? DBSCAN_HIP_OK : DBSCAN_HIP_ERR_N;
   if (!offsets || !eps || !min_samples) return DBSCAN_HIP_ERR_NULL;
   const int32_t B = static_cast<int32_t>(num_clouds);
-  std::vector<CloudInfo> info(B);
+  bt.info.assign(B, CloudInfo{});
   int64_t sort_tiles = 0, scan_tiles = 0;
   if (offsets[0] != 0 || offsets[B] != total) return DBSCAN_HIP_ERR_N;
   for (int32_t i = 0; i < B; ++i) {
-    CloudInfo& ci = info[i];
+    CloudInfo& ci = bt.info[i];
     std::memset(&ci, 0, sizeof(ci));
     if (offsets[i + 1] < offsets[i]) {
       if (bad_cloud) *bad_cloud = i;
@@ -4558,32 +4566,34 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
     sort_tiles += (ni + sort_tile - 1) / sort_tile;
     scan_tiles += (ni + kCellTile - 1) / kCellTile;
   }
-  if (total == 0) return DBSCAN_HIP_OK;
-  if (!X || !labels || !workspace) return DBSCAN_HIP_ERR_NULL;
-  if ((reinterpret_cast<uintptr_t>(X) & 7u) || (reinterpret_cast<uintptr_t>(labels) & 3u) ||
-      (reinterpret_cast<uintptr_t>(workspace) & 255u))
-    return DBSCAN_HIP_ERR_ALIGN;
-  const Layout L = make_layout(total, num_clouds);
-  if (workspace_bytes < L.total) return DBSCAN_HIP_ERR_WORKSPACE;
+  plan.total_points = total;
+  plan.num_clouds = num_clouds;
+  plan.sort_tiles = sort_tiles;
+  plan.scan_tiles = scan_tiles;
+  plan.cfg = cfg;
+  return DBSCAN_HIP_OK;
+}
 
-  auto fail = [&](hipError_t e) {
-    g_last_hip_error = static_cast<int>(e);
-    return DBSCAN_HIP_ERR_HIP;
-  };
+// Copies the staged tables into the workspace on `stream` and decides the
+// path: every cloud small, and at most kSmallGridClouds per CU, takes a
+// persistent grid of one-workgroup clusterings that draws the clouds from a
+// device ticket (allow_small false -- the per-kernel timing variant -- keeps
+// the pipeline).  Past that many clouds per CU the pipeline catches up (on
+// one MI355X, clouds of 200-4,096 points: 64 clouds 0.15 against 0.29 ms,
+// 1,024 clouds 0.35 against 0.46, 2,048 clouds 0.62 against 0.63).  The
+// host buffers are pageable, and a pageable host-to-device copy has read its
+// source when it returns, so they may go out of scope at once.
+int copy_batch_tables(BatchTables& bt, void* ws, hipStream_t stream, bool allow_small) {
+  dbscan_hip_batch_plan& plan = bt.plan;
+  const int32_t B = static_cast<int32_t>(plan.num_clouds);
+  const dbscan_hip_config& cfg = plan.cfg;
+  const Layout L = make_layout(plan.total_points, plan.num_clouds);
+  const std::vector<CloudInfo>& info = bt.info;
   hipError_t e;
-  void* ws = workspace;
-  const float2* X2 = reinterpret_cast<const float2*>(X);
-  Params* prm = at<Params>(ws, L.params);
-
-  // Every cloud small, and at most kSmallGridClouds per CU: a persistent grid
-  // of one-workgroup clusterings that takes the clouds from a device ticket
-  // (the per-kernel timing variant keeps the pipeline).  Past that many
-  // clouds per CU the pipeline catches up (on one MI355X, clouds of 200-4,096
-  // points: 64 clouds 0.15 against 0.29 ms, 1,024 clouds 0.35 against 0.46,
-  // 2,048 clouds 0.62 against 0.63).
-  bool all_small = kev == nullptr && cfg.small_max > 0 && B <= kSmallGridClouds * cu_count();
+  bool all_small = allow_small && cfg.small_max > 0 && B <= kSmallGridClouds * cu_count();
   for (int32_t i = 0; i < B && all_small; ++i)
     all_small = info[i].end - info[i].begin <= cfg.small_max;
+  plan.all_small = all_small ? 1 : 0;
   if (all_small) {
     // Largest clouds first (each cloud writes its labels at its own offset,
     // so the order is free): the workgroups that draw the last tickets then
@@ -4592,26 +4602,16 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
     std::stable_sort(order.begin(), order.end(), [](const CloudInfo& a, const CloudInfo& b) {
       return a.end - a.begin > b.end - b.begin;
     });
-    CloudInfo* clouds = at<CloudInfo>(ws, L.clouds);
-    if ((e = hipMemcpyAsync(clouds, order.data(), sizeof(CloudInfo) * B, hipMemcpyHostToDevice,
-                            stream)) != hipSuccess)
-      return fail(e);
-    const int grid = static_cast<int>(std::min<int64_t>(B, cu_count()));
-    const int32_t error0 = take_injected_error();
-    if (grid > 1) k_small_init<<<1, 64, 0, stream>>>(prm, error0);
-    k_small<<<grid, kSmallThreads, 0, stream>>>(X2, labels, clouds, B, CloudInfo{}, prm,
-                                                grid > 1 ? 0 : error0, error_out, total);
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
-    return DBSCAN_HIP_OK;
-  }
-
-  // Cloud records and tile tables, staged on the host and copied in one go
-  // (the copy is ordered before the kernels on `stream`).
-  std::vector<char> staging(L.seg_end - L.clouds, 0);
-  std::memcpy(staging.data(), info.data(), sizeof(CloudInfo) * B);
-  SegTile* st_sort = reinterpret_cast<SegTile*>(staging.data() + (L.seg_sort - L.clouds));
-  SegTile* st_scan = reinterpret_cast<SegTile*>(staging.data() + (L.seg_scan - L.clouds));
-  {
+    e = hipMemcpyAsync(at<CloudInfo>(ws, L.clouds), order.data(), sizeof(CloudInfo) * B,
+                       hipMemcpyHostToDevice, stream);
+  } else {
+    // Cloud records and tile tables, staged on the host and copied in one go
+    // (the copy is ordered before the kernels on `stream`).
+    const int32_t sort_tile = kSortThreads * cfg.sort_items;
+    std::vector<char> staging(L.seg_end - L.clouds, 0);
+    std::memcpy(staging.data(), info.data(), sizeof(CloudInfo) * B);
+    SegTile* st_sort = reinterpret_cast<SegTile*>(staging.data() + (L.seg_sort - L.clouds));
+    SegTile* st_scan = reinterpret_cast<SegTile*>(staging.data() + (L.seg_scan - L.clouds));
     int32_t a = 0, b = 0;
     for (int32_t i = 0; i < B; ++i) {
       const int32_t first = a;
@@ -4620,15 +4620,66 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
       for (int32_t p0 = info[i].begin; p0 < info[i].end; p0 += kCellTile)
         st_scan[b++] = SegTile{i, p0, std::min(kCellTile, info[i].end - p0), 0};
     }
+    e = hipMemcpyAsync(at<char>(ws, L.clouds), staging.data(), staging.size(),
+                       hipMemcpyHostToDevice, stream);
   }
+  if (e != hipSuccess) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  }
+  return DBSCAN_HIP_OK;
+}
 
+// The per-cloud cell bounds are accumulated with atomics by k_bounds_seg: a
+// launch sequence that may be replayed (a captured graph) starts from the
+// values the staging wrote.
+__global__ __launch_bounds__(256) void k_clouds_reset(CloudInfo* __restrict__ clouds,
+                                                      int32_t num_clouds) {
+  const int32_t i = static_cast<int32_t>(blockIdx.x) * 256 + static_cast<int32_t>(threadIdx.x);
+  if (i >= num_clouds) return;
+  CloudInfo& ci = clouds[i];
+  ci.kxmin = INT_MAX;
+  ci.kxmax = INT_MIN;
+  ci.kymin = INT_MAX;
+  ci.kymax = INT_MIN;
+}
+
+// Kernels only (no copy, allocation or host synchronisation): the batch
+// whose tables copy_batch_tables wrote into the workspace.  reset_bounds:
+// the sequence may run again on the same tables (dbscan2d_hip_batched_enqueue,
+// e.g. replays of a captured graph), so it first restores the per-cloud
+// bounds the staging wrote.
+int launch_batched(const float* X, int32_t* labels, void* workspace, hipStream_t stream,
+                   const dbscan_hip_batch_plan& plan, hipEvent_t* kev, int32_t* error_out,
+                   bool reset_bounds) {
+  const int64_t total = plan.total_points;
+  const int32_t B = static_cast<int32_t>(plan.num_clouds);
+  const dbscan_hip_config& cfg = plan.cfg;
+  const Layout L = make_layout(total, plan.num_clouds);
+  auto fail = [&](hipError_t e) {
+    g_last_hip_error = static_cast<int>(e);
+    return DBSCAN_HIP_ERR_HIP;
+  };
+  hipError_t e;
+  void* ws = workspace;
+  const float2* X2 = reinterpret_cast<const float2*>(X);
+  Params* prm = at<Params>(ws, L.params);
+  if (plan.all_small) {
+    CloudInfo* clouds = at<CloudInfo>(ws, L.clouds);
+    const int grid = static_cast<int>(std::min<int64_t>(B, cu_count()));
+    const int32_t error0 = take_injected_error();
+    if (grid > 1) k_small_init<<<1, 64, 0, stream>>>(prm, error0);
+    k_small<<<grid, kSmallThreads, 0, stream>>>(X2, labels, clouds, B, CloudInfo{}, prm,
+                                                grid > 1 ? 0 : error0, error_out, total);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    return DBSCAN_HIP_OK;
+  }
   auto before = [&](int k) -> hipError_t {
     return kev ? hipEventRecord(kev[k], stream) : hipSuccess;
   };
-  if ((e = hipMemcpyAsync(at<char>(ws, L.clouds), staging.data(), staging.size(),
-                          hipMemcpyHostToDevice, stream)) != hipSuccess)
-    return fail(e);
-
+  if (reset_bounds)
+    k_clouds_reset<<<(B + 255) / 256, 256, 0, stream>>>(at<CloudInfo>(ws, L.clouds), B);
+  const int64_t sort_tiles = plan.sort_tiles, scan_tiles = plan.scan_tiles;
   const int64_t n = total;
   uint64_t* sort_status = at<uint64_t>(ws, L.sort_status);
   uint32_t* cloud_hist = at<uint32_t>(ws, L.cloud_hist);
@@ -4753,6 +4804,22 @@ int run_batched(const float* X, int64_t total, const int64_t* offsets, const dou
   if ((e = before(kNumKernels)) != hipSuccess) return fail(e);
   if ((e = hipGetLastError()) != hipSuccess) return fail(e);
   return DBSCAN_HIP_OK;
+}
+int run_batched(const float* X, int64_t total, const int64_t* offsets, const double* eps,
+                const int64_t* min_samples, int64_t num_clouds, int32_t* labels,
+                void* workspace, size_t workspace_bytes, hipStream_t stream, hipEvent_t* kev,
+                int* bad_cloud, const dbscan_hip_config* cfgp, int32_t* error_out = nullptr) {
+  BatchTables bt;
+  const int rc = build_batch(total, offsets, eps, min_samples, num_clouds, bad_cloud, cfgp, bt);
+  if (rc != DBSCAN_HIP_OK || total == 0) return rc;
+  if (!X || !labels || !workspace) return DBSCAN_HIP_ERR_NULL;
+  if ((reinterpret_cast<uintptr_t>(X) & 7u) || (reinterpret_cast<uintptr_t>(labels) & 3u) ||
+      (reinterpret_cast<uintptr_t>(workspace) & 255u))
+    return DBSCAN_HIP_ERR_ALIGN;
+  if (workspace_bytes < make_layout(total, num_clouds).total) return DBSCAN_HIP_ERR_WORKSPACE;
+  const int rc2 = copy_batch_tables(bt, workspace, stream, kev == nullptr);
+  if (rc2 != DBSCAN_HIP_OK) return rc2;
+  return launch_batched(X, labels, workspace, stream, bt.plan, kev, error_out, false);
 }
 
 std::atomic<void*> g_scratch_addr[kMaxDevices];
@@ -4890,6 +4957,43 @@ int dbscan2d_hip_batched_timed(const float* X, int64_t total_points, const int64
   if (!events) return DBSCAN_HIP_ERR_NULL;
   return run_batched(X, total_points, offsets, eps, min_samples, num_clouds, labels, workspace,
                      workspace_bytes, stream, events, nullptr, cfg);
+}
+
+int dbscan2d_hip_batched_stage(int64_t total_points, const int64_t* offsets, const double* eps,
+                               const int64_t* min_samples, int64_t num_clouds, void* workspace,
+                               size_t workspace_bytes, hipStream_t stream, int* bad_cloud,
+                               const dbscan_hip_config* cfg, dbscan_hip_batch_plan* plan) {
+  if (!plan) return DBSCAN_HIP_ERR_NULL;
+  BatchTables bt;
+  const int rc = build_batch(total_points, offsets, eps, min_samples, num_clouds, bad_cloud, cfg,
+                             bt);
+  *plan = bt.plan;
+  if (rc != DBSCAN_HIP_OK || total_points == 0) return rc;
+  if (!workspace) return DBSCAN_HIP_ERR_NULL;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255u) return DBSCAN_HIP_ERR_ALIGN;
+  if (workspace_bytes < make_layout(total_points, num_clouds).total)
+    return DBSCAN_HIP_ERR_WORKSPACE;
+  const int rc2 = copy_batch_tables(bt, workspace, stream, true);
+  *plan = bt.plan;
+  return rc2;
+}
+
+int dbscan2d_hip_batched_enqueue(const float* X, int32_t* labels, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream,
+                                 const dbscan_hip_batch_plan* plan, int32_t* error_out) {
+  if (!plan) return DBSCAN_HIP_ERR_NULL;
+  const dbscan_hip_batch_plan& pl = *plan;
+  if (pl.total_points < 0 || pl.total_points >= INT32_MAX || pl.num_clouds < 0 ||
+      pl.num_clouds > (1 << 24) || !config_ok(pl.cfg))
+    return DBSCAN_HIP_ERR_CONFIG;
+  if (pl.total_points == 0) return DBSCAN_HIP_OK;
+  if (!X || !labels || !workspace) return DBSCAN_HIP_ERR_NULL;
+  if ((reinterpret_cast<uintptr_t>(X) & 7u) || (reinterpret_cast<uintptr_t>(labels) & 3u) ||
+      (reinterpret_cast<uintptr_t>(workspace) & 255u))
+    return DBSCAN_HIP_ERR_ALIGN;
+  if (workspace_bytes < make_layout(pl.total_points, pl.num_clouds).total)
+    return DBSCAN_HIP_ERR_WORKSPACE;
+  return launch_batched(X, labels, workspace, stream, pl, nullptr, error_out, true);
 }
 
 int dbscan2d_hip_layout(int64_t n, dbscan_hip_layout* out) {

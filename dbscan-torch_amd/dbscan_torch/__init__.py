@@ -58,7 +58,8 @@ _load_tuning()
 
 ArrayLike2D = Union[np.ndarray, torch.Tensor]
 
-__all__ = ["dbscan2d", "dbscan2d_batched", "DBSCAN", "ArrayLike2D", "check_device_errors"]
+__all__ = ["dbscan2d", "dbscan2d_batched", "BatchedGraph", "DBSCAN", "ArrayLike2D",
+           "check_device_errors"]
 
 try:
     __version__ = _pkg_version("dbscan-torch")
@@ -146,6 +147,73 @@ def dbscan2d_batched(clouds: Union[Sequence[torch.Tensor], torch.Tensor],
     labels = _C.dbscan2d_batched(packed, offs, _per_cloud(eps, B, "eps", float),
                                  _per_cloud(min_samples, B, "min_samples", int))
     return list(torch.split(labels, sizes))
+
+
+class BatchedGraph:
+    """A batched dbscan2d captured in a HIP graph once and replayed.
+
+    No counterpart in the reference.  ``X`` is one packed float32 ``(N, 2)``
+    tensor on a ROCm device and ``offsets``, ``eps`` and ``min_samples`` are as
+    for :func:`dbscan2d_batched`.  The batch's tables are staged into a
+    workspace of the object's own, one eager run checks the set-up, and the
+    kernels (no copy, allocation or host synchronisation among them) are
+    captured in a ``torch.cuda.CUDAGraph``.  The graph reads ``X`` and writes
+    :attr:`labels` in place, so refill ``X`` with other clouds of the same
+    sizes and call :meth:`replay` for their labels; :meth:`restage` changes
+    eps and min_samples for the same sizes.  Runs of one object must not
+    overlap (they share the workspace).
+    """
+
+    def __init__(self, X: torch.Tensor, offsets: Sequence[int],
+                 eps: Union[float, Sequence[float]], min_samples: Union[int, Sequence[int]]):
+        if not (isinstance(X, torch.Tensor) and X.is_cuda):
+            raise RuntimeError("BatchedGraph: X must be a tensor on a ROCm device")
+        self.X = X
+        self.offsets = [int(v) for v in (offsets.tolist() if hasattr(offsets, "tolist")
+                                         else offsets)]
+        B = len(self.offsets) - 1
+        dev = X.device
+        self.workspace = torch.empty(_C.dbscan2d_batched_workspace_bytes(int(X.size(0)), B),
+                                     dtype=torch.uint8, device=dev)
+        self.labels = torch.empty(int(X.size(0)), dtype=torch.int32, device=dev)
+        self.error_word = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.plan = None
+        self.restage(eps, min_samples)
+        # one eager run: kernels loaded, arguments checked on the device
+        _C.dbscan2d_batched_enqueue(self.X, self.labels, self.workspace, self.plan,
+                                    self.error_word)
+        self.check()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            _C.dbscan2d_batched_enqueue(self.X, self.labels, self.workspace, self.plan,
+                                        self.error_word)
+
+    def restage(self, eps: Union[float, Sequence[float]],
+                min_samples: Union[int, Sequence[int]]) -> None:
+        """Stages eps and min_samples (the cloud sizes stay those of the
+        capture); the next replay uses them."""
+        B = len(self.offsets) - 1
+        plan = _C.dbscan2d_batched_stage(self.X, self.offsets, _per_cloud(eps, B, "eps", float),
+                                         _per_cloud(min_samples, B, "min_samples", int),
+                                         self.workspace)
+        if self.plan is not None and plan != self.plan:
+            raise RuntimeError("BatchedGraph.restage: the staged launch plan changed; "
+                               "capture a new BatchedGraph")
+        self.plan = plan
+        torch.cuda.current_stream(self.X.device).synchronize()
+
+    def replay(self) -> torch.Tensor:
+        """Runs the captured batch on the current contents of X; returns
+        :attr:`labels` (packed int32, written in stream order)."""
+        self.graph.replay()
+        return self.labels
+
+    def check(self) -> None:
+        """Waits for the last run and raises RuntimeError if it failed a
+        device-side check."""
+        bits = int(self.error_word.item())
+        if bits:
+            _C.raise_for_device_error(bits)
 
 
 class DBSCAN:

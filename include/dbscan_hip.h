@@ -195,6 +195,42 @@ int dbscan2d_hip_batched_checked(const float* X, int64_t total_points, const int
                                  const dbscan_hip_config* cfg, int32_t* error_out);
 
 /*
+ * dbscan2d_hip_batched in two steps, so that the launches can be captured in
+ * a HIP graph once per batch shape and replayed (no reference counterpart).
+ *
+ * dbscan2d_hip_batched_stage validates the batch as dbscan2d_hip_batched
+ * does, copies its per-cloud records and tile tables into `workspace` with
+ * hipMemcpyAsync on `stream` (the host arrays may be reused when it
+ * returns), and fills *plan with what the launches need.
+ * dbscan2d_hip_batched_enqueue then enqueues kernels only -- no copy,
+ * allocation, memset or host synchronisation -- on `stream`: the clustering
+ * of the points at X into labels (both device memory, as for
+ * dbscan2d_hip_batched), with the staged tables; error_out as for
+ * dbscan2d_hip_batched_checked.  It may be captured with
+ * hipStreamBeginCapture and the graph replayed any number of times: each
+ * run starts from the staged tables, so a replay after X is refilled with
+ * other clouds of the same sizes, eps and min_samples gives their labels.
+ * Restage (same workspace) to change eps or min_samples but not the sizes;
+ * a batch of other sizes needs its own stage, plan and graph.  Runs that
+ * share a workspace must not overlap.  Labels equal dbscan2d_hip_batched's.
+ */
+typedef struct {
+  int64_t total_points, num_clouds;
+  int64_t sort_tiles, scan_tiles;  /* launch sizes */
+  int32_t all_small;               /* 1: the one-workgroup persistent grid */
+  int32_t reserved_;
+  dbscan_hip_config cfg;
+} dbscan_hip_batch_plan;
+
+int dbscan2d_hip_batched_stage(int64_t total_points, const int64_t* offsets, const double* eps,
+                               const int64_t* min_samples, int64_t num_clouds, void* workspace,
+                               size_t workspace_bytes, hipStream_t stream, int* bad_cloud,
+                               const dbscan_hip_config* cfg, dbscan_hip_batch_plan* plan);
+int dbscan2d_hip_batched_enqueue(const float* X, int32_t* labels, void* workspace,
+                                 size_t workspace_bytes, hipStream_t stream,
+                                 const dbscan_hip_batch_plan* plan, int32_t* error_out);
+
+/*
  * dbscan2d_hip_batched with the event slots of dbscan2d_hip_timed (slot 0
  * also covers the per-cloud key-map kernel).  No counterpart in the
  * reference: per-kernel timing of batches.

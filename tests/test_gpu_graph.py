@@ -57,3 +57,59 @@ def test_graph_replays_equal_eager_calls(dt):
         torch.cuda.synchronize()
         assert torch.equal(gout, eager2), k
     assert np.array_equal(eager2.cpu().numpy(), oracle.dbscan2d(X2, 0.5, 10))
+
+
+def _packed(sizes, seed0):
+    clouds = [W.cloud_at_density(n, seed0 + i) for i, n in enumerate(sizes)]
+    offs = [0]
+    for c in clouds:
+        offs.append(offs[-1] + len(c))
+    return clouds, offs, np.concatenate(clouds)
+
+
+@pytest.mark.parametrize("sizes", [
+    # a pipeline batch (clouds above the one-workgroup limit among them)
+    [3_000, 9_000, 150_000, 40_000, 700],
+    # every cloud small: the persistent one-workgroup grid
+    [300, 4_096, 1_200, 2_500],
+], ids=["pipeline", "all_small"])
+def test_batched_graph_replays_equal_eager_batches(dt, sizes):
+    eps = [0.3, 0.5, 0.7, 0.4, 0.6][:len(sizes)]
+    ms = [5, 10, 20, 10, 5][:len(sizes)]
+    clouds, offs, packed = _packed(sizes, 500)
+    Xd = torch.from_numpy(packed).to(DEV)
+    eager = dt.dbscan2d_batched(Xd, eps, ms, offsets=offs)
+    for i in (0, len(sizes) - 1):  # the oracle, independently of the batched path
+        assert np.array_equal(eager[offs[i]:offs[i + 1]].cpu().numpy(),
+                              oracle.dbscan2d(clouds[i], eps[i], ms[i])), i
+    print(f"[gpu graph] batched capture sizes={sizes}", flush=True)
+    g = dt.BatchedGraph(Xd, offs, eps, ms)
+    other = torch.from_numpy(W.cloud_at_density(30_000, 81)).to(DEV)
+    for k in range(4):
+        g.labels.fill_(-7)
+        out = g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), k
+        g.check()
+        dt.dbscan2d(other, 0.3, 5)  # eager work between the replays
+        dt.dbscan2d_batched([other, other[:1000]], 0.5, 10)
+        torch.cuda.synchronize()
+    # other clouds of the same sizes, refilled in place
+    clouds2, _, packed2 = _packed(sizes, 900)
+    Xd.copy_(torch.from_numpy(packed2))
+    eager2 = dt.dbscan2d_batched(Xd, eps, ms, offsets=offs)
+    for k in range(2):
+        out = g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager2), k
+    assert np.array_equal(eager2[offs[0]:offs[1]].cpu().numpy(),
+                          oracle.dbscan2d(clouds2[0], eps[0], ms[0]))
+    # new eps and min_samples for the same sizes
+    eps3, ms3 = [e * 0.8 for e in eps], [m + 1 for m in ms]
+    eager3 = dt.dbscan2d_batched(Xd, eps3, ms3, offsets=offs)
+    g.restage(eps3, ms3)
+    for k in range(2):
+        out = g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager3), k
+    g.check()
