@@ -738,6 +738,11 @@ __device__ __forceinline__ uint64_t digit_lookback(const uint64_t* __restrict__ 
 // then reordered by digit through one LDS buffer, array by array, so that runs
 // of equal digits leave as contiguous writes.  Thread t owns digits
 // [t*kDigitsPerThread, (t+1)*kDigitsPerThread).
+// Every pass but the last writes (x, y) as 8-byte pairs over the first 8n
+// bytes of its x buffer (the y array follows the x array in the workspace,
+// so the pairs fit) and the next pass reads them back: one store per point
+// instead of two, and fewer partly written cache lines at the ends of the
+// digit runs.  The last pass writes the x and y arrays the cell stages read.
 // kSeg (batches): tile t is tiles[t], a run of one
 // cloud's points; the cloud's own histogram, key map and scale apply, the
 // look-back stops at the cloud's first tile, and the cloud's points stay in
@@ -827,7 +832,8 @@ __device__ __forceinline__ void sort_pass_tile(
         const float2 p = X[ec];
         x[k] = p.x; y[k] = p.y; id[k] = e;
       } else {
-        x[k] = in_x[ec]; y[k] = in_y[ec]; id[k] = in_i[ec];
+        const float2 p = reinterpret_cast<const float2*>(in_x)[ec];  // an earlier pass's pairs
+        x[k] = p.x; y[k] = p.y; id[k] = in_i[ec];
       }
     }
     // all loads of the tile are in flight before the first key is computed
@@ -926,15 +932,28 @@ __device__ __forceinline__ void sort_pass_tile(
       const int j = t + q * kSortThreads;
       pos[q] = j < static_cast<int>(tile_n) ? s_goff[s_d[j]] + static_cast<uint32_t>(j) : n;
     }
+    if (pass + 1 == prm->npasses) {
 #pragma unroll
-    for (int q = 0; q < kItems; ++q) {
-      const int j = t + q * kSortThreads;
-      if (pos[q] < n) {
-        const uint64_t v = s_buf[j];
-        out_x[pos[q]] = __uint_as_float(static_cast<uint32_t>(v));
-        out_y[pos[q]] = __uint_as_float(static_cast<uint32_t>(v >> 32));
-      } else if (j < static_cast<int>(tile_n)) {
-        atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+      for (int q = 0; q < kItems; ++q) {
+        const int j = t + q * kSortThreads;
+        if (pos[q] < n) {
+          const uint64_t v = s_buf[j];
+          out_x[pos[q]] = __uint_as_float(static_cast<uint32_t>(v));
+          out_y[pos[q]] = __uint_as_float(static_cast<uint32_t>(v >> 32));
+        } else if (j < static_cast<int>(tile_n)) {
+          atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+        }
+      }
+    } else {
+      uint64_t* __restrict__ out_xy = reinterpret_cast<uint64_t*>(out_x);
+#pragma unroll
+      for (int q = 0; q < kItems; ++q) {
+        const int j = t + q * kSortThreads;
+        if (pos[q] < n) {
+          out_xy[pos[q]] = s_buf[j];
+        } else if (j < static_cast<int>(tile_n)) {
+          atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
+        }
       }
     }
     __syncthreads();
