@@ -31,7 +31,10 @@ min_samples 10, seed 2).  Prints one JSON line:
                  dbscan2d calls over the same resident clouds, labels compared
   serving        open-loop Poisson request streams through the continuous-
                  batching server: p50/p99 latency and clouds/s per rate, and
-                 serial plain calls on the same requests
+                 serial plain calls on the same requests; host-origin requests
+                 (points in pinned host memory, labels returned there) and,
+                 under device_resident, the same requests with their points
+                 already in HBM and their labels left there
 """
 
 from __future__ import annotations
@@ -356,7 +359,8 @@ def small_leg(dbscan_torch) -> dict:
     return out
 
 
-def serving_leg(dbscan_torch, rates, count: int, pool: int = 384) -> dict:
+def serving_leg(dbscan_torch, rates, count: int, pool: int = 384,
+                dev_rates=(1000, 2000, 5000)) -> dict:
     """Open-loop Poisson request streams served by the continuous-batching
     server (pooled arena: 3 slots of 8M points for requests below 1M points,
     3 slots of 24M points for the larger ones): p50 / p99 latency and
@@ -364,7 +368,8 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 384) -> dict:
     requests.  Request i of a stream draws eps_i and min_samples_i from the
     stream and its points from a pool of `pool` pre-generated pinned clouds
     whose sizes follow the N mixture (generating every request's points on
-    the host would take minutes)."""
+    the host would take minutes).  The streams at `dev_rates` and a burst
+    are also served with the pool's clouds resident in HBM."""
     import numpy as np
     import torch
 
@@ -435,6 +440,59 @@ def serving_leg(dbscan_torch, rates, count: int, pool: int = 384) -> dict:
     out["burst_points"] = int(sum(r.n for r in reqs[:k]))
     out["labels_identical_to_plain"] = all(torch.equal(p, r.labels) for p, r in
                                            zip(plain, server_reqs))
+    out["burst_speedup_vs_serial"] = round(serial / burst, 2)
+
+    # The same streams with every request's points already in HBM (produced
+    # by an upstream GPU stage): the server copies them device to device and
+    # leaves their labels on the device, as dbscan2d does, so nothing crosses
+    # PCIe.  Against serial plain dbscan2d calls on the same device tensors.
+    dclouds = [c.to("cuda") for c in clouds]
+    torch.cuda.synchronize()
+
+    def make_dev(rate):
+        arr, reqs = make(rate)
+        for r in reqs:
+            r.X = dclouds[r.rid % pool]
+        return arr, reqs
+
+    dev = {"rates": {}}
+    for rate in dev_rates:
+        arr, reqs = make_dev(rate)
+        nb = len(server.batches)
+        run_open_loop(server, reqs, arr)
+        summ = latency_summary(reqs)
+        b = server.batches[nb:]
+        summ.update({"offered_per_s": rate, "batches": len(b),
+                     "mean_batch": round(sum(b) / max(1, len(b)), 2)})
+        dev["rates"][str(rate)] = summ
+        print(f"[bench] serving device-resident rate {rate}: {summ}", file=sys.stderr, flush=True)
+    arr, reqs = make_dev(rates[0])
+    for r in reqs[:3]:
+        dbscan_torch.dbscan2d(r.X, r.eps, r.min_samples)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dplain = [dbscan_torch.dbscan2d(r.X, r.eps, r.min_samples) for r in reqs[:k]]
+    torch.cuda.synchronize()
+    dserial = time.perf_counter() - t0
+    bursts = []
+    for rep in range(4):
+        burst_reqs = [Request(r.rid, r.X, r.eps, r.min_samples) for r in reqs[:k]]
+        server_reqs = run_open_loop(server, burst_reqs, [0.0] * k)
+        if rep:
+            bursts.append(max(r.done for r in server_reqs) - min(r.arrival for r in server_reqs))
+    burst = sorted(bursts)[1]
+    dev["serial_plain_clouds_per_s"] = round(k / dserial, 1)
+    dev["burst_served_clouds_per_s"] = round(k / burst, 1)
+    dev["burst_ms"] = [round(b * 1e3, 2) for b in bursts]
+    dev["burst_speedup_vs_serial"] = round(dserial / burst, 2)
+    dev["labels_identical_to_plain"] = all(
+        torch.equal(p, r.labels) and torch.equal(h, r.labels.cpu())
+        for p, r, h in zip(dplain, server_reqs, plain))
+    print(f"[bench] serving device-resident: serial {dev['serial_plain_clouds_per_s']} "
+          f"clouds/s, burst {dev['burst_served_clouds_per_s']} clouds/s", file=sys.stderr,
+          flush=True)
+    out["device_resident"] = dev
+    del dclouds, dplain
     return out
 
 

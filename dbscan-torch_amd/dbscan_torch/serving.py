@@ -47,11 +47,13 @@ def _align(n: int) -> int:
 @dataclasses.dataclass
 class Request:
     """One dbscan2d call.  ``X`` is a float32 (N, 2) tensor, on the host
-    (pinned memory lets its copy overlap) or on the server's device.
-    ``arrival`` is the time the request is due (clock seconds); latency is
-    measured from it to the moment its labels are in host memory.  ``error``
-    is None, or the text of a failed device-side check of its batch (its
-    labels are then INT32_MIN)."""
+    (pinned memory lets its copy overlap) or on the server's device.  Its
+    ``labels`` end up where ``X`` is, as ``dbscan2d`` returns them: in host
+    memory for a host ``X``, on the device for a device-resident one (nothing
+    crosses PCIe then).  ``arrival`` is the time the request is due (clock
+    seconds); latency is measured from it to the moment its labels are
+    complete in that memory.  ``error`` is None, or the text of a failed
+    device-side check of its batch (its labels are then INT32_MIN)."""
 
     rid: int
     X: torch.Tensor
@@ -74,10 +76,13 @@ class Request:
 
 class _Slot:
     """One batch's share of the arena: points, labels, workspace and its
-    stream.  A batch's labels go device-to-host in one copy into a pinned
-    tensor of the batch's own (torch's caching host allocator); each request
-    receives its slice of it, so retiring a batch copies nothing on the host
-    (a request's labels keep that batch's buffer alive)."""
+    stream.  A batch's labels leave the arena in at most two copies: one
+    device-to-host copy into a pinned tensor of the batch's own (torch's
+    caching host allocator) when it holds host requests, and one
+    device-to-device copy into a device tensor of its own when it holds
+    device-resident requests.  Each request receives its slice of one of
+    them, so retiring a batch copies nothing on the host (a request's labels
+    keep that batch's buffer alive)."""
 
     def __init__(self, arena: torch.Tensor, offset: int, points: int, clouds: int,
                  ws_bytes: int, device: torch.device, big: bool = False):
@@ -93,6 +98,7 @@ class _Slot:
         self.batch: List[Request] = []
         self.offsets: List[int] = []
         self.host_labels: Optional[torch.Tensor] = None
+        self.dev_labels: Optional[torch.Tensor] = None
 
     @staticmethod
     def nbytes(points: int, ws_bytes: int) -> int:
@@ -174,7 +180,7 @@ class Server:
             raise ValueError(f"Server: request {req.rid} has {req.n} points; a slot holds "
                              f"{limit}")
         if req.n == 0:
-            req.labels = torch.empty(0, dtype=torch.int32)
+            req.labels = torch.empty(0, dtype=torch.int32, device=X.device)
             req.admitted = req.done = self.clock()
             self.completed[req.rid] = req
             return
@@ -205,24 +211,32 @@ class Server:
         now = self.clock()
         for r in batch:
             r.admitted = now
-        host = torch.empty(pts, dtype=torch.int32, pin_memory=True)
+        on_device = [r.X.is_cuda for r in batch]
+        host = (torch.empty(pts, dtype=torch.int32, pin_memory=True)
+                if not all(on_device) else None)
+        dev = None
         # device-resident inputs may still be being written on the caller's
         # stream: the slot's stream waits for it, and their memory is kept
         # until the slot's copies have read it
-        if any(r.X.is_cuda for r in batch):
+        if any(on_device):
             slot.stream.wait_stream(torch.cuda.current_stream(self.device))
-            for r in batch:
-                if r.X.is_cuda:
+            for r, d in zip(batch, on_device):
+                if d:
                     r.X.record_stream(slot.stream)
         with torch.cuda.stream(slot.stream):
             _C.copy_clouds(slot.X, [r.X for r in batch], offs[:-1])
             _C.dbscan2d_batched_out(slot.X[:pts], offs, [r.eps for r in batch],
                                     [r.min_samples for r in batch], slot.labels,
                                     slot.workspace, slot.err)
-            host.copy_(slot.labels[:pts], non_blocking=True)
+            if host is not None:
+                host.copy_(slot.labels[:pts], non_blocking=True)
+            if any(on_device):
+                # allocated on the slot's stream, which writes it; it is handed
+                # out only after the batch's event has completed
+                dev = slot.labels[:pts].clone()
             slot.event = torch.cuda.Event()
             slot.event.record(slot.stream)
-        slot.batch, slot.offsets, slot.host_labels = batch, offs, host
+        slot.batch, slot.offsets, slot.host_labels, slot.dev_labels = batch, offs, host, dev
         self.inflight.append(slot)
         self.batches.append(len(batch))
 
@@ -231,13 +245,14 @@ class Server:
         bits = int(slot.err[0])
         why = _C.device_error_message(bits) if bits else None
         for r, o in zip(slot.batch, slot.offsets):
-            r.labels = slot.host_labels[o:o + r.n]
+            r.labels = (slot.dev_labels if r.X.is_cuda else slot.host_labels)[o:o + r.n]
             r.done = now
             r.error = why
             if why:
                 self.failed.append(r.rid)
             self.completed[r.rid] = r
-        slot.batch, slot.offsets, slot.host_labels, slot.event = [], [], None, None
+        slot.batch, slot.offsets, slot.event = [], [], None
+        slot.host_labels = slot.dev_labels = None
         self.free.append(slot)
 
     def step(self) -> bool:

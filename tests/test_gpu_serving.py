@@ -89,7 +89,26 @@ def test_device_resident_requests(srv):
         server.submit(r)
     server.drain()
     for r in reqs:
-        assert np.array_equal(r.labels.numpy(),
+        # labels stay on the device of X, as dbscan2d returns them
+        assert r.labels.device == r.X.device
+        assert np.array_equal(r.labels.cpu().numpy(),
+                              oracle.dbscan2d(r.X.cpu().numpy(), r.eps, r.min_samples)), r.rid
+
+
+def test_mixed_host_and_device_requests_in_one_batch(srv):
+    """One batch holding host and device-resident requests: each request's
+    labels come back where its points are, and equal the oracle's."""
+    server = srv.Server(slots=1, slot_points=2 << 20, slot_clouds=32, cap_bytes=4e9)
+    reqs = _requests(srv, 20, 6, pinned=True)
+    for i, r in enumerate(reqs):
+        if i % 2:
+            r.X = r.X.to(DEV)
+        server.submit(r)
+    server.drain()
+    assert server.batches == [sum(r.n > 0 for r in reqs)]
+    for r in reqs:
+        assert r.error is None and r.labels.device == r.X.device, r.rid
+        assert np.array_equal(r.labels.cpu().numpy(),
                               oracle.dbscan2d(r.X.cpu().numpy(), r.eps, r.min_samples)), r.rid
 
 
@@ -167,7 +186,8 @@ def test_device_inputs_written_asynchronously(srv):
     server.drain()
     for r in reqs:
         assert r.error is None
-        assert np.array_equal(r.labels.numpy(), oracle.dbscan2d(r.X.cpu().numpy(), 0.5, 5)), r.rid
+        assert r.labels.is_cuda
+        assert np.array_equal(r.labels.cpu().numpy(), oracle.dbscan2d(r.X.cpu().numpy(), 0.5, 5)), r.rid
 
 
 def test_size_aware_admission(srv):
