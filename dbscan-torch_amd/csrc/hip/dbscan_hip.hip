@@ -213,11 +213,13 @@ struct CellParams {
   int32_t ms;
 };
 struct OneCloud {
+  static constexpr bool kUniform = true;  // every cell has the same parameters
   float eps_sq, best_init;
   int32_t ms;
   __device__ __forceinline__ CellParams at(int32_t) const { return {eps_sq, best_init, ms}; }
 };
 struct ManyClouds {
+  static constexpr bool kUniform = false;
   const int32_t* __restrict__ cell_cloud;
   const CloudInfo* __restrict__ clouds;
   __device__ __forceinline__ CellParams at(int32_t c) const {
@@ -1879,56 +1881,54 @@ __device__ __forceinline__ void count_range(const float* __restrict__ xs,
   }
 }
 
-// Pass-2 count for one point by one lane (dbscan_grid.hpp:284-307): the own
-// cell counts wholesale (`total` on entry); window cells whose box is farther
-// than eps add nothing, cells whose box lies within eps add all their points,
-// small cells are scanned.  Sets `deferred` when a big cell would need a scan.
-__device__ __forceinline__ int32_t lane_core_count(
+// Pass-2 count for one point by one lane (dbscan_grid.hpp:284-307), one
+// window column [j0, j1) at a time: window cells whose box is farther than
+// eps add nothing, cells whose box lies within eps add all their points,
+// small cells are scanned (the own cell c was counted wholesale by the
+// caller).  Sets `deferred` when a big cell would need a scan.
+__device__ __forceinline__ int32_t lane_column_count(
     const float* __restrict__ xs, const float* __restrict__ ys, float x, float y, int32_t c,
-    const int32_t* __restrict__ row, const int32_t* __restrict__ cell_start,
+    int32_t j0, int32_t j1, const int32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ cell_flags, const uint4* __restrict__ cell_box,
     int32_t min_samples, float eps_sq, int32_t total, bool& deferred) {
-  for (int k = 0; k < 5 && total < min_samples; ++k) {
-    const int col = near_col(k);
-    WinColumn w;
-    load_column(cell_start, cell_flags, cell_box, row[2 * col], row[2 * col + 1], w);
-    // the column's cells nearest box first (the count stops at min_samples):
-    // cells within eps wholesale, then the partial ones by box distance
-    // (eps_sq may be +inf: the cells still to scan are a bit mask, not a
-    // distance sentinel)
-    float d[5];
-    uint32_t todo = 0;
+  WinColumn w;
+  load_column(cell_start, cell_flags, cell_box, j0, j1, w);
+  // the column's cells nearest box first (the count stops at min_samples):
+  // cells within eps wholesale, then the partial ones by box distance
+  // (eps_sq may be +inf: the cells still to scan are a bit mask, not a
+  // distance sentinel)
+  float d[5];
+  uint32_t todo = 0;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      d[q] = 0.f;
-      if (q >= w.cnt || w.j0 + q == c) continue;
-      d[q] = point_box_d2(x, y, w.box[q]);
-      if (d[q] > eps_sq) continue;
-      if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq))
-        total += w.start[q + 1] - w.start[q];
-      else
-        todo |= 1u << q;
-    }
+  for (int q = 0; q < 5; ++q) {
+    d[q] = 0.f;
+    if (q >= w.cnt || w.j0 + q == c) continue;
+    d[q] = point_box_d2(x, y, w.box[q]);
+    if (d[q] > eps_sq) continue;
+    if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq))
+      total += w.start[q + 1] - w.start[q];
+    else
+      todo |= 1u << q;
+  }
 #pragma unroll
-    for (int it = 0; it < 5; ++it) {
-      if (!todo || total >= min_samples) break;
-      int qm = -1;
+  for (int it = 0; it < 5; ++it) {
+    if (!todo || total >= min_samples) break;
+    int qm = -1;
 #pragma unroll
-      for (int q = 0; q < 5; ++q)
-        if (((todo >> q) & 1u) && (qm < 0 || d[q] < d[qm])) qm = q;
-      todo &= ~(1u << qm);
-      int32_t cs = 0, ce = 0;
+    for (int q = 0; q < 5; ++q)
+      if (((todo >> q) & 1u) && (qm < 0 || d[q] < d[qm])) qm = q;
+    todo &= ~(1u << qm);
+    int32_t cs = 0, ce = 0;
 #pragma unroll
-      for (int q = 0; q < 5; ++q)
-        if (q == qm) {
-          cs = w.start[q];
-          ce = w.start[q + 1];
-        }
-      if (ce - cs <= kLaneScan)
-        total = lane_count_range(xs, ys, cs, ce, x, y, eps_sq, min_samples, total);
-      else
-        deferred = true;
-    }
+    for (int q = 0; q < 5; ++q)
+      if (q == qm) {
+        cs = w.start[q];
+        ce = w.start[q + 1];
+      }
+    if (ce - cs <= kLaneScan)
+      total = lane_count_range(xs, ys, cs, ce, x, y, eps_sq, min_samples, total);
+    else
+      deferred = true;
   }
   return total;
 }
@@ -1983,13 +1983,17 @@ __device__ __forceinline__ int64_t cells_per_wave(int64_t C) {
 }
 
 // Core marking pass 2 (dbscan_grid.hpp:281-307) for the cells with fewer than
-// min_samples points (pass 1 was done by k_bbox).  One lane per cell counts
-// for each of the cell's points in turn; the counts stop at min_samples,
+// min_samples points (pass 1 was done by k_bbox).  A wave takes up to 64
+// cells (one lane each), prunes the cells whose whole window holds fewer than
+// min_samples points, and counts for the remaining cells' points from a work
+// list, one window column per lane and step; the counts stop at min_samples,
 // which cannot change the outcome.  Points whose count needs a scan of a big
 // cell, and cells of more than 64 points (possible when min_samples > 64),
-// are counted by the whole wave.  Sets the cell's core bits.
+// are counted by the whole wave.  Sets the cell's core bits, once per cell.
+// Launch bounds hold it to 72 VGPRs (7 waves per SIMD; 74 uncapped gave 6:
+// 6% faster on skewed10m).
 template <class PP>
-__global__ __launch_bounds__(256) void k_core_sparse(
+__global__ __launch_bounds__(256, 7) void k_core_sparse(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
     const float* __restrict__ ys_a, const float* __restrict__ xs_b,
     const float* __restrict__ ys_b, const int32_t* __restrict__ cell_start,
@@ -1999,10 +2003,15 @@ __global__ __launch_bounds__(256) void k_core_sparse(
   const bool in_a = (prm->npasses & 1) != 0;
   const float* __restrict__ xs = in_a ? xs_a : xs_b;
   const float* __restrict__ ys = in_a ? ys_a : ys_b;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
   const int64_t K = cells_per_wave(C);
+  // per wave: the window row of each lane's point (column-major, so that the
+  // lanes' words fall in distinct banks) and the core / non-core bits of the
+  // cell each lane owns, gathered here and written once per cell
+  __shared__ int32_t s_row[4][10][64];
+  __shared__ uint32_t s_bits[4][64];
   for (int64_t base = wave * K; base < C; base += nwaves * K) {
     const int32_t c = static_cast<int32_t>(base + lane);
     const bool valid = lane < K && base + lane < C;
@@ -2013,6 +2022,7 @@ __global__ __launch_bounds__(256) void k_core_sparse(
       e = cell_start[c + 1];
       cp = pp.at(c);
     }
+    s_bits[wib][lane] = 0u;
     const int32_t min_samples = cp.ms;
     const float eps_sq = cp.eps_sq;
     const bool sparse = valid && e - s < min_samples;
@@ -2037,41 +2047,91 @@ __global__ __launch_bounds__(256) void k_core_sparse(
         bits = kCellHasNonCore;
       }
     }
-    // The points of the cells still undecided are shared out one per lane:
-    // work item q of the wave is point q - excl of the cell of the lane whose
-    // [excl, incl) holds q (a binary search over the lanes' prefix sums).
+    // The points of the cells still undecided form the wave's work list:
+    // item q is point q - excl of the cell of the lane whose [excl, incl)
+    // holds q (a binary search over the lanes' prefix sums).  A lane takes an
+    // item and counts for it one window column per step, nearest column
+    // first.  A lane whose point is decided (min_samples reached, or all five
+    // columns seen) takes the next item in the same step, so every step
+    // costs one column's loads for the lanes at work, instead of a round of
+    // 64 points waiting for the lane whose point needs the most columns.
     const uint32_t cnt = (sparse && !whole_wave && !lonely) ? static_cast<uint32_t>(e - s) : 0u;
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
-    for (uint32_t r = 0; r < total; r += 64) {
-      const uint32_t q = r + static_cast<uint32_t>(lane);
-      int o = 0;  // owner lane: the number of lanes with incl <= q
+    uint32_t next = 0;  // the first item not yet taken (wave-uniform)
+    bool need = true, active = false, deferred = false;
+    int32_t p = 0, oc = 0, oms = 0, tot = 0, k = 0, ol = 0;
+    float px = 0.f, py = 0.f, oeps = 0.f;
+    if constexpr (PP::kUniform) {  // kernel arguments, no per-lane copies
+      oms = pp.at(0).ms;
+      oeps = pp.at(0).eps_sq;
+    }
+    while (next < total || __ballot(active)) {
+      const uint64_t nm = __ballot(need);
+      if (nm && next >= total) {
+        need = false;  // nothing left to take: these lanes are done
+      } else if (nm) {
+        const uint32_t q = next + static_cast<uint32_t>(
+                                      __popcll(nm & ((1ull << lane) - 1ull)));
+        next += static_cast<uint32_t>(__popcll(nm));
+        int o = 0;  // owner lane of item q: the number of lanes with incl <= q
 #pragma unroll
-      for (int step = 32; step >= 1; step >>= 1) {
-        const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(incl), o + step - 1, 64));
-        if (v <= q) o += step;
+        for (int step = 32; step >= 1; step >>= 1) {
+          const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(incl), o + step - 1, 64));
+          if (v <= q) o += step;
+        }
+        o = min(o, 63);
+        const int32_t c_o = __shfl(c, o, 64), s_o = __shfl(s, o, 64), e_o = __shfl(e, o, 64);
+        const uint32_t x_o = static_cast<uint32_t>(__shfl(static_cast<int>(incl - cnt), o, 64));
+        int32_t ms_o = 0;
+        float eps_o = 0.f;
+        if constexpr (!PP::kUniform) {
+          ms_o = __shfl(min_samples, o, 64);
+          eps_o = __shfl(eps_sq, o, 64);
+        }
+        if (need) {
+          need = false;
+          active = q < total;
+          if (active) {
+            ol = o;
+            oc = c_o;
+            if constexpr (!PP::kUniform) {
+              oms = ms_o;
+              oeps = eps_o;
+            }
+            p = s_o + static_cast<int32_t>(q - x_o);
+            tot = e_o - s_o;  // the own cell, wholesale
+            k = 0;
+            deferred = false;
+            px = xs[p];
+            py = ys[p];
+            const int32_t* row = ngh + static_cast<int64_t>(oc) * 10;
+#pragma unroll
+            for (int i = 0; i < 10; ++i) s_row[wib][i][lane] = row[i];
+          }
+        }
       }
-      o = min(o, 63);
-      const int32_t oc = __shfl(c, o, 64), os = __shfl(s, o, 64), oe = __shfl(e, o, 64);
-      const uint32_t oex = static_cast<uint32_t>(__shfl(static_cast<int>(incl - cnt), o, 64));
-      const int32_t oms = __shfl(min_samples, o, 64);
-      const float oeps = __shfl(eps_sq, o, 64);
-      bool deferred = false;
-      int32_t p = 0, tot = 0;
-      if (q < total) {
-        p = os + static_cast<int32_t>(q - oex);
-        const int32_t* row = ngh + static_cast<int64_t>(oc) * 10;
-        tot = lane_core_count(xs, ys, xs[p], ys[p], oc, row, cell_start, cell_flags, cell_box,
-                              oms, oeps, oe - os, deferred);
-        if (!deferred || tot >= oms) {
-          const bool core = tot >= oms;
-          is_core[p] = core ? 1 : 0;
-          atomicOr(cell_flags + oc, core ? kCellHasCore : kCellHasNonCore);
-          deferred = false;
+      bool finished = false;
+      if (active) {
+        const int col = near_col(k);
+        const int32_t j0 = s_row[wib][2 * col][lane], j1 = s_row[wib][2 * col + 1][lane];
+        tot = lane_column_count(xs, ys, px, py, oc, j0, j1, cell_start, cell_flags, cell_box, oms,
+                                oeps, tot, deferred);
+        ++k;
+        if (tot >= oms || k == 5) {
+          finished = true;
+          active = false;
+          need = true;
+          if (!deferred || tot >= oms) {
+            const bool core = tot >= oms;
+            is_core[p] = core ? 1 : 0;
+            atomicOr(&s_bits[wib][ol], core ? kCellHasCore : kCellHasNonCore);
+            deferred = false;
+          }
         }
       }
       // points whose count needs a scan of a big cell: the whole wave, one by one
-      uint64_t dm = __ballot(deferred);
+      uint64_t dm = __ballot(finished && deferred);
       while (dm) {
         const int src = lowest_lane(dm);
         dm &= dm - 1ull;
@@ -2081,13 +2141,19 @@ __global__ __launch_bounds__(256) void k_core_sparse(
         const int32_t* row = ngh + static_cast<int64_t>(pc) * 10;
         const int32_t wt = wave_core_count(xs, ys, xs[pp_], ys[pp_], pc, row, cell_start,
                                            cell_flags, cell_box, pms, peps,
-                                           bcast(oe - os, src));
+                                           cell_start[pc + 1] - cell_start[pc]);
         if (lane == src) {
           const bool core = wt >= pms;
           is_core[pp_] = core ? 1 : 0;
-          atomicOr(cell_flags + pc, core ? kCellHasCore : kCellHasNonCore);
+          atomicOr(&s_bits[wib][ol], core ? kCellHasCore : kCellHasNonCore);
         }
       }
+    }
+    // the wave's own LDS writes, in order: every point's bits are in
+    __builtin_amdgcn_wave_barrier();
+    if (cnt) {
+      const uint32_t b = s_bits[wib][lane];
+      if (b) atomicOr(cell_flags + c, b);
     }
     // cells of more than 64 points (min_samples > 64): the whole wave
     uint64_t todo = __ballot(whole_wave);
