@@ -2919,74 +2919,65 @@ __global__ __launch_bounds__(kThreads) void k_unpermute(
   for (uint32_t j = b0 + t; j < b1; j += kThreads) labels[j] = failed ? kPoison : s_lab[j - b0];
 }
 
-// Border search for one point by one lane (dbscan_grid.hpp:419-445): over
-// the window cells holding a core with a box within eps, the first strict
-// minimum of d2 over cores with d2 <= eps_sq in the reference's scan order
-// wins.  Sets `deferred` when a big cell would need a scan.
-__device__ __forceinline__ int32_t lane_border(
+// One window column [j0, j1) (column `col`, scan indices col * 5 + q) of a
+// non-core point's border search by one lane.  The column's cells are
+// visited by box distance, and a cell whose box is farther than the best d2
+// so far cannot hold a better or tying core, so the rest of the column is
+// skipped.  best / best_idx / lab keep the smallest (d2, scan index): the
+// reference's first strict minimum in its scan order (dx-major, ky
+// ascending), whatever the visiting order; all cores of a cell share its
+// label.  Sets `deferred` when a big cell would need a scan.
+__device__ __forceinline__ void lane_border_column(
     const float* __restrict__ xs, const float* __restrict__ ys,
-    const uint8_t* __restrict__ is_core, float x, float y, const int32_t* __restrict__ row,
+    const uint8_t* __restrict__ is_core, float x, float y, int col, int32_t j0, int32_t j1,
     const int32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_flags,
     const uint4* __restrict__ cell_box, const int32_t* __restrict__ cell_cluster, float eps_sq,
-    float best_init, bool& deferred) {
-  // Cells are visited nearest first (columns from the point's own outwards,
-  // a column's cells by box distance), and a cell whose box is farther than
-  // the best d2 so far cannot hold a better or tying core, so the rest of
-  // the column is skipped.  The winner is the smallest (d2, scan index):
-  // the reference's first strict minimum in its scan order (dx-major, ky
-  // ascending), whatever the visiting order; all cores of a cell share its
-  // label.
-  float best = best_init;
-  int32_t lab = kNoise, best_idx = 1 << 30;
-  for (int k = 0; k < 5; ++k) {
-    const int col = near_col(k);
-    WinColumn w;
-    load_column(cell_start, cell_flags, cell_box, row[2 * col], row[2 * col + 1], w);
-    float d[5];
-    uint32_t todo = 0;
+    float& best, int32_t& best_idx, int32_t& lab, bool& deferred) {
+  WinColumn w;
+  load_column(cell_start, cell_flags, cell_box, j0, j1, w);
+  float d[5];
+  uint32_t todo = 0;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      d[q] = 0.f;
-      if (q >= w.cnt || !(w.flags[q] & kCellHasCore)) continue;
-      d[q] = point_box_d2(x, y, w.box[q]);
-      if (d[q] > eps_sq) continue;
-      todo |= 1u << q;
-    }
+  for (int q = 0; q < 5; ++q) {
+    d[q] = 0.f;
+    if (q >= w.cnt || !(w.flags[q] & kCellHasCore)) continue;
+    d[q] = point_box_d2(x, y, w.box[q]);
+    if (d[q] > eps_sq) continue;
+    todo |= 1u << q;
+  }
 #pragma unroll
-    for (int it = 0; it < 5; ++it) {
-      if (!todo) break;
-      int qm = -1;
+  for (int it = 0; it < 5; ++it) {
+    if (!todo) break;
+    int qm = -1;
 #pragma unroll
-      for (int q = 0; q < 5; ++q)
-        if (((todo >> q) & 1u) && (qm < 0 || d[q] < d[qm])) qm = q;
-      todo &= ~(1u << qm);
-      float dq = 0.f;
-      int32_t cs = 0, ce = 0;
+    for (int q = 0; q < 5; ++q)
+      if (((todo >> q) & 1u) && (qm < 0 || d[q] < d[qm])) qm = q;
+    todo &= ~(1u << qm);
+    float dq = 0.f;
+    int32_t cs = 0, ce = 0;
 #pragma unroll
-      for (int q = 0; q < 5; ++q)
-        if (q == qm) {
-          dq = d[q];
-          cs = w.start[q];
-          ce = w.start[q + 1];
-        }
-      if (dq > best) break;  // and every cell after it in this column
-      if (ce - cs > kLaneScan) {
-        deferred = true;
-        continue;
+    for (int q = 0; q < 5; ++q)
+      if (q == qm) {
+        dq = d[q];
+        cs = w.start[q];
+        ce = w.start[q + 1];
       }
-      const uint32_t cm = lane_min_core_d2(xs, ys, is_core, cs, ce, x, y, eps_sq);
-      const int32_t idx = col * 5 + qm;
-      if (cm != 0xFFFFFFFFu) {
-        const float v = __uint_as_float(cm);
-        if (v < best || (v == best && idx < best_idx)) {
-          best = v;
-          best_idx = idx;
-          lab = cell_cluster[w.j0 + qm];
-        }
+    if (dq > best) break;  // and every cell after it in this column
+    if (ce - cs > kLaneScan) {
+      deferred = true;
+      continue;
+    }
+    const uint32_t cm = lane_min_core_d2(xs, ys, is_core, cs, ce, x, y, eps_sq);
+    const int32_t idx = col * 5 + qm;
+    if (cm != 0xFFFFFFFFu) {
+      const float v = __uint_as_float(cm);
+      if (v < best || (v == best && idx < best_idx)) {
+        best = v;
+        best_idx = idx;
+        lab = cell_cluster[w.j0 + qm];
       }
     }
   }
-  return lab;
 }
 
 // The same search by the whole wave for the point (xi, yi): each candidate
@@ -3032,10 +3023,11 @@ __device__ int32_t wave_border(const float* __restrict__ xs, const float* __rest
 }
 
 // Border assignment (dbscan_grid.hpp:415-446) for the cells that hold a
-// non-core point: the points of the wave's cells are shared out one per lane;
-// a lane searches for its non-core point and writes the label in sorted order
-// into the index buffer the sort left spare (k_label moves it on).  Points
-// whose search needs a big cell are searched by the whole wave.
+// non-core point: a cell with no core cell in its window labels its points
+// noise at once; the other cells' non-core points are searched 12 per step,
+// five lanes per point (one window column each), and the label goes in
+// sorted order into the index buffer the sort left spare (k_label moves it
+// on).  Points whose search needs a big cell are searched by the whole wave.
 template <class PP>
 __global__ __launch_bounds__(256) void k_border(
     const Params* __restrict__ prm, const float* __restrict__ xs_a,
@@ -3086,14 +3078,18 @@ __global__ __launch_bounds__(256) void k_border(
       }
     }
     const float eps_sq = cp.eps_sq, best_init = cp.best_init;
-    // The points of these cells are shared out one per lane (work item q of
-    // the wave is point q - excl of the cell of the lane whose [excl, incl)
-    // holds q); core points are skipped.
+    // The points of these cells are searched 12 per step, each by 5 lanes,
+    // one window column per lane (lanes 60-63 idle): point item q of the
+    // wave is point q - excl of the cell of the lane whose [excl, incl)
+    // holds q.  The five columns' loads go out together, and the lanes'
+    // results are merged into the smallest (d2, scan index).  Core points
+    // are skipped.
     const uint32_t cnt = has ? static_cast<uint32_t>(e - s) : 0u;
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
-    for (uint32_t r = 0; r < total; r += 64) {
-      const uint32_t q = r + static_cast<uint32_t>(lane);
+    const int grp = lane / 5, col = lane - 5 * grp;
+    for (uint32_t r = 0; r < total; r += 12) {
+      const uint32_t q = r + static_cast<uint32_t>(grp);
       int o = 0;  // owner lane: the number of lanes with incl <= q
 #pragma unroll
       for (int step = 32; step >= 1; step >>= 1) {
@@ -3104,29 +3100,47 @@ __global__ __launch_bounds__(256) void k_border(
       const int32_t oc = __shfl(c, o, 64), os = __shfl(s, o, 64);
       const uint32_t oex = static_cast<uint32_t>(__shfl(static_cast<int>(incl - cnt), o, 64));
       const float oeps = __shfl(eps_sq, o, 64), obest = __shfl(best_init, o, 64);
-      bool deferred = false;
-      int32_t p = 0;
-      if (q < total) {
+      bool deferred = false, work = false;
+      int32_t p = 0, lab = kNoise, best_idx = 1 << 30;
+      float best = obest;
+      if (lane < 60 && q < total) {
         p = os + static_cast<int32_t>(q - oex);
-        if (!is_core[p]) {
+        work = !is_core[p];
+        if (work) {
           const int32_t* row = ngh + static_cast<int64_t>(oc) * 10;
-          const int32_t lab = lane_border(xs, ys, is_core, xs[p], ys[p], row, cell_start,
-                                          cell_flags, cell_box, cell_cluster, oeps, obest,
-                                          deferred);
-          if (!deferred) lab_sorted[p] = lab;
+          lane_border_column(xs, ys, is_core, xs[p], ys[p], col, row[2 * col], row[2 * col + 1],
+                             cell_start, cell_flags, cell_box, cell_cluster, oeps, best,
+                             best_idx, lab, deferred);
         }
       }
+      // merge the point's five columns into its column-0 lane
+#pragma unroll
+      for (int dd = 1; dd <= 4; dd <<= 1) {
+        const float bo = __shfl_down(best, dd, 64);
+        const int32_t io = __shfl_down(best_idx, dd, 64), lo = __shfl_down(lab, dd, 64);
+        const bool fo = __shfl_down(static_cast<int>(deferred), dd, 64) != 0;
+        if (col + dd < 5) {
+          if (bo < best || (bo == best && io < best_idx)) {
+            best = bo;
+            best_idx = io;
+            lab = lo;
+          }
+          deferred = deferred || fo;
+        }
+      }
+      const bool lead = work && col == 0;
+      if (lead && !deferred) lab_sorted[p] = lab;
       // points whose search needs a big cell: the whole wave, one by one
-      uint64_t dm = __ballot(deferred);
+      uint64_t dm = __ballot(lead && deferred);
       while (dm) {
         const int src = lowest_lane(dm);
         dm &= dm - 1ull;
         const int32_t pc = bcast(oc, src), pp_ = bcast(p, src);
         const int32_t* row = ngh + static_cast<int64_t>(pc) * 10;
-        const int32_t lab = wave_border(xs, ys, is_core, xs[pp_], ys[pp_], row, cell_start,
-                                        cell_flags, cell_box, cell_cluster, bcastf(oeps, src),
-                                        bcastf(obest, src));
-        if (lane == src) lab_sorted[pp_] = lab;
+        const int32_t wl = wave_border(xs, ys, is_core, xs[pp_], ys[pp_], row, cell_start,
+                                       cell_flags, cell_box, cell_cluster, bcastf(oeps, src),
+                                       bcastf(obest, src));
+        if (lane == src) lab_sorted[pp_] = wl;
       }
     }
   }
