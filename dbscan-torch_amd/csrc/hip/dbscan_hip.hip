@@ -2282,30 +2282,52 @@ __global__ __launch_bounds__(256, 8) void k_union(
     const int64_t t = base + lane;
     int32_t c = -1, nc = -1;
     int slot = 0;
+    // Three rounds of loads: the lane's own cell with its window run for
+    // `slot`, then the neighbour, then the point samples.
+    int32_t sa = 0, ea = 0, sb = 0, eb = 0;
+    float4 box_c = make_float4(0.f, 0.f, 0.f, 0.f), box_n = box_c;
     if (t < total) {
       c = static_cast<int32_t>(t / S);
       slot = static_cast<int>(t - static_cast<int64_t>(c) * S);
-      if (cell_flags[c] & kCellHasCore) {
+      const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+      const int ia = slot < 2 ? 5 : (slot < 7 ? 6 : 8);
+      const int32_t ra = row[ia], rb = row[slot < 2 ? 5 : ia + 1];
+      const uint32_t fl_c = cell_flags[c];
+      const uint32_t ky_c = static_cast<uint32_t>(cell_key[c]);
+      sa = cell_start[c];
+      ea = cell_start[c + 1];
+      box_c = load_box(cell_box, c);
+      if (fl_c & kCellHasCore) {
+        // forward neighbour `slot`: the next two cells of c's column, then
+        // the runs of the two columns to the right
         int dx;
-        nc = forward_neighbour(ngh + static_cast<int64_t>(c) * 10, c, slot, dx);
+        if (slot < 2) {
+          dx = 0;
+          nc = c + 1 + slot < ra ? c + 1 + slot : -1;
+        } else {
+          dx = slot < 7 ? 1 : 2;
+          const int32_t k = ra + (slot - (slot < 7 ? 2 : 7));
+          nc = k < rb ? k : -1;
+        }
         if (nc >= 0) {
-          const int64_t dy = static_cast<int64_t>(static_cast<uint32_t>(cell_key[nc])) -
-                             static_cast<int64_t>(static_cast<uint32_t>(cell_key[c]));
-          if ((dx <= 1 && dy >= -1 && dy <= 1) != (kPhase == 0)) nc = -1;
+          const uint32_t ky_n = static_cast<uint32_t>(cell_key[nc]);
+          const uint32_t fl_n = cell_flags[nc];
+          box_n = load_box(cell_box, nc);
+          sb = cell_start[nc];
+          eb = cell_start[nc + 1];
+          const int64_t dy = static_cast<int64_t>(ky_n) - static_cast<int64_t>(ky_c);
+          if ((dx <= 1 && dy >= -1 && dy <= 1) != (kPhase == 0) || !(fl_n & kCellHasCore))
+            nc = -1;
         }
       }
     }
     // after the first phase's flatten, uf holds roots: equal roots need no test
-    if (nc >= 0 && !(cell_flags[nc] & kCellHasCore)) nc = -1;
     if (kPhase == 1 && nc >= 0 && uf[cell_ref[c]] == uf[cell_ref[nc]]) nc = -1;
     bool deferred = false;
     float eps_sq = 0.f;  // c and nc are cells of one cloud
     if (nc >= 0) {
       eps_sq = pp.at(c).eps_sq;
-      const float4 box_c = load_box(cell_box, c), box_n = load_box(cell_box, nc);
       if (box_box_d2(box_c, box_n) <= eps_sq) {
-        const int32_t sa = cell_start[c], ea = cell_start[c + 1];
-        const int32_t sb = cell_start[nc], eb = cell_start[nc + 1];
         const bool hit = core_pair_sample(xs, ys, is_core, sa, ea, sb, eb, eps_sq);
         deferred = !hit;
         if (hit) {
