@@ -1577,6 +1577,10 @@ __device__ __forceinline__ void bbox_tile(const Params* __restrict__ prm,
       p.mnx = p.mxx = f2ord(px[j]);
       p.mny = p.mxy = f2ord(py[j]);
     } else {
+      // the box ignores NaN points; its x maximum becomes +inf, so that no
+      // box test takes the cell as lying within eps (the other tests only
+      // weaken: a bigger box is never farther than a point of it)
+      p.mxx = f2ord(__builtin_inff());
       p.fl = kCellHasNaN;
     }
     if (full[j]) p.fl |= kCellHasCore;
@@ -1826,11 +1830,11 @@ __device__ __forceinline__ float far_box_d2(float x, float y, float4 b) {
   return dx * dx + dy * dy;
 }
 
-// Every point of cell nc is within eps of (x, y): box test, finite, no NaN point.
-__device__ __forceinline__ bool cell_inside(float x, float y, float4 b, uint32_t flags,
-                                            float eps_sq) {
+// Every point of cell nc is within eps of (x, y): box test, finite.  A cell
+// holding a NaN point has +inf as its box's x maximum (k_bbox), so it fails.
+__device__ __forceinline__ bool cell_inside(float x, float y, float4 b, float eps_sq) {
   const float far = far_box_d2(x, y, b);
-  return !(flags & kCellHasNaN) && far <= eps_sq && far < __builtin_inff();
+  return far <= eps_sq && far < __builtin_inff();
 }
 
 // The cells of one window column: at most 5 consecutive cells [j0, j0 + cnt)
@@ -1843,6 +1847,7 @@ struct WinColumn {
   float4 box[5];
 };
 
+template <bool kFlags = true>
 __device__ __forceinline__ void load_column(const int32_t* __restrict__ cell_start,
                                             const uint32_t* __restrict__ cell_flags,
                                             const uint4* __restrict__ cell_box, int32_t j0,
@@ -1853,7 +1858,7 @@ __device__ __forceinline__ void load_column(const int32_t* __restrict__ cell_sta
   for (int k = 0; k < 6; ++k) w.start[k] = (k <= w.cnt && w.cnt > 0) ? cell_start[j0 + k] : 0;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    w.flags[k] = k < w.cnt ? cell_flags[j0 + k] : 0u;
+    if constexpr (kFlags) w.flags[k] = k < w.cnt ? cell_flags[j0 + k] : 0u;
     w.box[k] = k < w.cnt ? load_box(cell_box, j0 + k) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
@@ -1892,7 +1897,7 @@ __device__ __forceinline__ int32_t lane_column_count(
     const uint32_t* __restrict__ cell_flags, const uint4* __restrict__ cell_box,
     int32_t min_samples, float eps_sq, int32_t total, bool& deferred) {
   WinColumn w;
-  load_column(cell_start, cell_flags, cell_box, j0, j1, w);
+  load_column<false>(cell_start, cell_flags, cell_box, j0, j1, w);  // flags unused
   // the column's cells nearest box first (the count stops at min_samples):
   // cells within eps wholesale, then the partial ones by box distance
   // (eps_sq may be +inf: the cells still to scan are a bit mask, not a
@@ -1905,7 +1910,7 @@ __device__ __forceinline__ int32_t lane_column_count(
     if (q >= w.cnt || w.j0 + q == c) continue;
     d[q] = point_box_d2(x, y, w.box[q]);
     if (d[q] > eps_sq) continue;
-    if (cell_inside(x, y, w.box[q], w.flags[q], eps_sq))
+    if (cell_inside(x, y, w.box[q], eps_sq))
       total += w.start[q + 1] - w.start[q];
     else
       todo |= 1u << q;
@@ -1957,7 +1962,7 @@ __device__ int32_t wave_core_count(const float* __restrict__ xs, const float* __
       if (own) continue;
       const float4 b = load_box(cell_box, nc);
       if (point_box_d2(xi, yi, b) > eps_sq) continue;
-      if (cell_inside(xi, yi, b, cell_flags[nc], eps_sq)) {
+      if (cell_inside(xi, yi, b, eps_sq)) {
         total += ce - cs;
         continue;
       }
