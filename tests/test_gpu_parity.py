@@ -89,6 +89,28 @@ def test_random_inputs_match_oracle(dt, seed):
     assert np.array_equal(gpu_labels(dt, X, eps, ms), oracle.dbscan2d(X, eps, ms))
 
 
+@pytest.mark.parametrize("ms", [65, 100, 300])
+def test_min_samples_above_64(dt, ms):
+    """min_samples above 64: cells of 65 to ms - 1 points are sparse and
+    larger than a lane's scan, so core marking counts their points with the
+    whole wave and border assignment searches them with the wave; labels
+    equal the oracle's on the pipeline (63,000 points), on the one-workgroup
+    path (one blob of 3,000 points) and in a batch of both."""
+    rng = np.random.default_rng(100 + ms)
+    blobs = np.concatenate([rng.normal(c, 0.3, (3000, 2)) for c in rng.uniform(0, 30, (20, 2))])
+    X = np.concatenate([blobs, rng.uniform(-2, 32, (3000, 2))]).astype(np.float32)
+    X = X[rng.permutation(len(X))]
+    small = blobs[:3000].astype(np.float32)  # one blob: dense cells in a small cloud
+    want, want_small = oracle.dbscan2d(X, 0.5, ms), oracle.dbscan2d(small, 0.5, ms)
+    assert (want >= 0).any() and (want == -1).any()
+    assert np.array_equal(gpu_labels(dt, X, 0.5, ms), want)
+    assert np.array_equal(gpu_labels(dt, small, 0.5, ms), want_small)
+    out = dt.dbscan2d_batched([torch.from_numpy(X).to(DEV), torch.from_numpy(small).to(DEV)],
+                              [0.5, 0.5], [ms, ms])
+    assert np.array_equal(out[0].cpu().numpy(), want)
+    assert np.array_equal(out[1].cpu().numpy(), want_small)
+
+
 def test_blobs10k_golden(dt, golden_dir, golden):
     X, eps, ms = W.blobs10k()
     labels = gpu_labels(dt, X, eps, ms)
