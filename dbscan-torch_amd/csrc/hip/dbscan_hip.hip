@@ -2244,21 +2244,20 @@ constexpr int kUnionSlots = 12;
 // Cells per workgroup of k_union_local; its ranges start at multiples of it.
 constexpr int kUnionRange = 1024;
 
-// Forward neighbour `slot` of cell c from its window row, with its column
-// offset, or -1.  Slots 0-1: the next two cells of c's column; 2-6: the cells
-// of the column to the right; 7-11: the column after that.
-__device__ __forceinline__ int32_t forward_neighbour(const int32_t* __restrict__ row, int32_t c,
-                                                     int slot, int& dx) {
+// Forward neighbour `slot` of cell c, with its column offset, or -1.  Slots
+// 0-1: the next two cells of c's column (ra = the end of c's window run in
+// its own column, window-row word 5); 2-6: the cells of the column to the
+// right, 7-11: the column after that ([ra, rb) = that column's window run,
+// words 6-7 or 8-9).
+__device__ __forceinline__ int32_t forward_neighbour(int32_t c, int slot, int32_t ra, int32_t rb,
+                                                     int& dx) {
   if (slot < 2) {
     dx = 0;
-    return c + 1 + slot < row[5] ? c + 1 + slot : -1;
+    return c + 1 + slot < ra ? c + 1 + slot : -1;
   }
-  if (slot < 7) {
-    dx = 1;
-    return row[6] + (slot - 2) < row[7] ? row[6] + (slot - 2) : -1;
-  }
-  dx = 2;
-  return row[8] + (slot - 7) < row[9] ? row[8] + (slot - 7) : -1;
+  dx = slot < 7 ? 1 : 2;
+  const int32_t k = ra + (slot - (slot < 7 ? 2 : 7));
+  return k < rb ? k : -1;
 }
 
 // Launch bounds hold it to 64 VGPRs (8 waves per SIMD; 66 uncapped gave 7):
@@ -2290,6 +2289,7 @@ __global__ __launch_bounds__(256, 8) void k_union(
     // Three rounds of loads: the lane's own cell with its window run for
     // `slot`, then the neighbour, then the point samples.
     int32_t sa = 0, ea = 0, sb = 0, eb = 0;
+    int32_t root_c = -1, ref_n = -1;  // kPhase 1: c's root, nc's reference index
     float4 box_c = make_float4(0.f, 0.f, 0.f, 0.f), box_n = box_c;
     if (t < total) {
       c = static_cast<int32_t>(t / S);
@@ -2302,19 +2302,16 @@ __global__ __launch_bounds__(256, 8) void k_union(
       sa = cell_start[c];
       ea = cell_start[c + 1];
       box_c = load_box(cell_box, c);
+      int32_t ref_c = 0;
+      if constexpr (kPhase == 1) ref_c = cell_ref[c];
       if (fl_c & kCellHasCore) {
-        // forward neighbour `slot`: the next two cells of c's column, then
-        // the runs of the two columns to the right
         int dx;
-        if (slot < 2) {
-          dx = 0;
-          nc = c + 1 + slot < ra ? c + 1 + slot : -1;
-        } else {
-          dx = slot < 7 ? 1 : 2;
-          const int32_t k = ra + (slot - (slot < 7 ? 2 : 7));
-          nc = k < rb ? k : -1;
-        }
+        nc = forward_neighbour(c, slot, ra, rb, dx);
         if (nc >= 0) {
+          if constexpr (kPhase == 1) {
+            root_c = uf[ref_c];  // with the neighbour's loads
+            ref_n = cell_ref[nc];
+          }
           const uint32_t ky_n = static_cast<uint32_t>(cell_key[nc]);
           const uint32_t fl_n = cell_flags[nc];
           box_n = load_box(cell_box, nc);
@@ -2327,7 +2324,7 @@ __global__ __launch_bounds__(256, 8) void k_union(
       }
     }
     // after the first phase's flatten, uf holds roots: equal roots need no test
-    if (kPhase == 1 && nc >= 0 && uf[cell_ref[c]] == uf[cell_ref[nc]]) nc = -1;
+    if (kPhase == 1 && nc >= 0 && root_c == uf[ref_n]) nc = -1;
     bool deferred = false;
     float eps_sq = 0.f;  // c and nc are cells of one cloud
     if (nc >= 0) {
@@ -2434,12 +2431,22 @@ __global__ __launch_bounds__(kUnionLocalThreads) void k_union_local(
     for (int i = t; i < nloc; i += kUnionLocalThreads) {
       const int32_t c = static_cast<int32_t>(c0) + i;
       uint32_t m = hits[c];
+      // the 3x3 slots (0-6) need only the window row's words 5-7: loaded
+      // once for the cell, not once per hit
+      int32_t r5 = 0, r6 = 0, r7 = 0;
+      if (m) {
+        const int32_t* row = ngh + static_cast<int64_t>(c) * 10;
+        r5 = row[5];
+        r6 = row[6];
+        r7 = row[7];
+      }
       while (m) {
         const int slot = __builtin_ctz(m);
         m &= m - 1u;
         int dx;
-        const int32_t nc = forward_neighbour(ngh + static_cast<int64_t>(c) * 10, c, slot, dx);
-        if (nc < 0) {  // a hit is only recorded for an existing neighbour
+        const int32_t nc = slot < 2 ? forward_neighbour(c, slot, r5, r5, dx)
+                                    : forward_neighbour(c, slot, r6, r7, dx);
+        if (nc < 0 || slot >= 7) {  // a hit is only recorded for an existing neighbour
           atomicOr(&prm->error, static_cast<int32_t>(kErrRange));
         } else if (nc < c1) {
           lds_unite(s_par, s_ref, i, nc - static_cast<int32_t>(c0), &prm->error);
