@@ -2012,9 +2012,10 @@ __global__ __launch_bounds__(256, 7) void k_core_sparse(
   const int64_t wave = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
   const int64_t K = cells_per_wave(C);
-  // per wave: the window row of each lane's point (column-major, so that the
-  // lanes' words fall in distinct banks) and the core / non-core bits of the
-  // cell each lane owns, gathered here and written once per cell
+  // per wave: the window row of the cell each lane owns (column-major, so
+  // that the lanes' words fall in distinct banks; read by the lanes counting
+  // its points) and the core / non-core bits of that cell, gathered here and
+  // written once per cell
   __shared__ int32_t s_row[4][10][64];
   __shared__ uint32_t s_bits[4][64];
   for (int64_t base = wave * K; base < C; base += nwaves * K) {
@@ -2047,11 +2048,15 @@ __global__ __launch_bounds__(256, 7) void k_core_sparse(
       for (int k = 0; k < 5; ++k)
         wt += j[2 * k + 1] > j[2 * k] ? cell_start[j[2 * k + 1]] - cell_start[j[2 * k]] : 0;
       lonely = wt < min_samples;
+      // the row, for the lanes that will count this cell's points
+#pragma unroll
+      for (int k = 0; k < 10; ++k) s_row[wib][k][lane] = j[k];
       if (lonely) {
         for (int32_t p = s; p < e; ++p) is_core[p] = 0;
         bits = kCellHasNonCore;
       }
     }
+    __builtin_amdgcn_wave_barrier();  // the owners' rows are in LDS
     // The points of the cells still undecided form the wave's work list:
     // item q is point q - excl of the cell of the lane whose [excl, incl)
     // holds q (a binary search over the lanes' prefix sums).  A lane takes an
@@ -2108,18 +2113,15 @@ __global__ __launch_bounds__(256, 7) void k_core_sparse(
             tot = e_o - s_o;  // the own cell, wholesale
             k = 0;
             deferred = false;
-            px = xs[p];
+            px = xs[p];  // used only after the first column's loads are issued
             py = ys[p];
-            const int32_t* row = ngh + static_cast<int64_t>(oc) * 10;
-#pragma unroll
-            for (int i = 0; i < 10; ++i) s_row[wib][i][lane] = row[i];
           }
         }
       }
       bool finished = false;
       if (active) {
         const int col = near_col(k);
-        const int32_t j0 = s_row[wib][2 * col][lane], j1 = s_row[wib][2 * col + 1][lane];
+        const int32_t j0 = s_row[wib][2 * col][ol], j1 = s_row[wib][2 * col + 1][ol];
         tot = lane_column_count(xs, ys, px, py, oc, j0, j1, cell_start, cell_flags, cell_box, oms,
                                 oeps, tot, deferred);
         ++k;
